@@ -1,0 +1,226 @@
+"""Synthetic flood cases written in the reference's input formats.
+
+Every case is written as a .par file plus ESRI ASCII rasters (DEM and start
+depth) and, where needed, a .bci boundary/point-source file and a .bdy time
+series file, so the drop-in file path (input/read_raster_file.cpp,
+classes/Boundary.h, classes/PointSources.h) is what loads them.
+
+Randomness comes from a SplitMix64 stream turned into normals with the
+Box-Muller transform, so a seed gives the same case on any host.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n successive SplitMix64 outputs for a seed (uint64)."""
+    with np.errstate(over="ignore"):
+        gamma = np.uint64(0x9E3779B97F4A7C15)
+        s = np.uint64(seed) + gamma * np.arange(1, n + 1, dtype=np.uint64)
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """Uniform doubles in [0, 1) from the top 53 bits."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """Standard normals by Box-Muller on pairs of uniforms."""
+    m = (n + 1) // 2
+    u = uniform(seed, 2 * m)
+    u1 = 1.0 - u[0::2]                      # (0, 1]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    return z[:n]
+
+
+def gaussian_random_field(n: int, seed: int, std: float, power: float = -3.0) -> np.ndarray:
+    """Isotropic GRF on an n x n grid with spectrum |k|^power, rescaled to std."""
+    w = normal(seed, n * n).reshape(n, n)
+    fk = np.fft.fft2(w)
+    kx = np.fft.fftfreq(n)
+    k = np.sqrt(kx[None, :] ** 2 + kx[:, None] ** 2)
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** (power / 2.0)
+    f = np.real(np.fft.ifft2(fk * amp))
+    f -= f.mean()
+    s = f.std()
+    return f * (std / s) if s > 0 else f
+
+
+def write_raster(path: str, a: np.ndarray, xll: float, yll: float, cell: float, fmt: str = "%.9g") -> None:
+    """ESRI ASCII grid; a[j, i] with j = 0 the southern row (written last)."""
+    nrows, ncols = a.shape
+    with open(path, "w") as f:
+        f.write(f"ncols        {ncols}\nnrows        {nrows}\nxllcorner    {xll!r}\n"
+                f"yllcorner    {yll!r}\ncellsize     {cell!r}\nNODATA_value -9999\n")
+        np.savetxt(f, a[::-1, :], fmt=fmt)
+
+
+def write_par(path: str, entries: list) -> None:
+    with open(path, "w") as f:
+        for e in entries:
+            f.write((e if isinstance(e, str) else f"{e[0]:<14}{e[1]}") + "\n")
+
+
+def _par(solver: str, L: int, eps: float, t_end: float, manning: float, extra=()) -> list:
+    ent = ["test_case     0", solver, ("max_ref_lvl", L), ("epsilon", repr(eps)),
+           ("wall_height", "1000"), ("initial_tstep", "1"), ("fpfric", repr(manning)),
+           ("sim_time", repr(t_end)), ("massint", repr(t_end)), ("saveint", repr(t_end)),
+           ("DEMfile", "case.dem"), ("startfile", "case.start")]
+    ent.extend(extra)
+    return ent
+
+
+def circular_dam_break(d: str, L: int = 8, eps: float = 1e-3, t_end: float = 3.5,
+                       solver: str = "mwdg2", cfl: float | None = 0.33) -> str:
+    """Circular dam break on a flat, frictionless 50 m square with closed walls."""
+    os.makedirs(d, exist_ok=True)
+    n = 1 << L
+    dx = 50.0 / n
+    c = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(c, c)
+    h = np.where((X - 25.0) ** 2 + (Y - 25.0) ** 2 < 2.5 ** 2, 2.5, 0.5)
+    write_raster(os.path.join(d, "case.dem"), np.zeros((n, n)), 0.0, 0.0, dx)
+    write_raster(os.path.join(d, "case.start"), h, 0.0, 0.0, dx)
+    extra = [("cfl", repr(cfl))] if cfl is not None else []
+    par = os.path.join(d, "case.par")
+    write_par(par, _par(solver, L, eps, t_end, 0.0, extra))
+    return par
+
+
+def hydrograph_series(width: float) -> list:
+    """Inflow per unit width (m^2/s) for 0->50->50->0 m^3/s at 0/600/1800/3600 s."""
+    return [(0.0, 0.0), (50.0 / width, 600.0), (50.0 / width, 1800.0), (0.0, 3600.0)]
+
+
+def wetdry_hydrograph(d: str, L: int = 10, eps: float = 1e-3, t_end: float = 3600.0,
+                      solver: str = "mwdg2", seed: int = 2, std: float = 5.0,
+                      manning: float = 0.035, peak: float = 50.0) -> str:
+    """Dry GRF bed with a W->E slope, QVAR inflow on the west edge, FREE east edge."""
+    os.makedirs(d, exist_ok=True)
+    n = 1 << L
+    dx = 1.0
+    z = gaussian_random_field(n, seed, std)
+    x = (np.arange(n) + 0.5) * dx
+    z = z + 0.001 * (n * dx - x)[None, :]
+    write_raster(os.path.join(d, "case.dem"), z, 0.0, 0.0, dx)
+    write_raster(os.path.join(d, "case.start"), np.zeros((n, n)), 0.0, 0.0, dx)
+    W = n * dx
+    with open(os.path.join(d, "case.bci"), "w") as f:
+        f.write(f"W 0 {W!r} QVAR inflow\nE 0 {W!r} FREE\n")
+    series = [(v * peak / 50.0, t) for v, t in hydrograph_series(W)]
+    with open(os.path.join(d, "case.bdy"), "w") as f:
+        f.write(f"inflow\n{len(series)} seconds\n")
+        for v, t in series:
+            f.write(f"{v!r} {t!r}\n")
+    par = os.path.join(d, "case.par")
+    write_par(par, _par(solver, L, eps, t_end, manning, [("bcifile", "case.bci"), ("bdyfile", "case.bdy")]))
+    return par
+
+
+def urban(d: str, L: int = 12, eps: float = 1e-3, t_end: float = 1800.0, solver: str = "mwdg2",
+          seed: int = 3, src_seed: int = 33, cover: float = 0.30, manning: float = 0.02,
+          n_src: int = 16, q_src: float = 5.0) -> str:
+    """Flat plain with seeded rectangular blocks, a wet northern strip and point sources."""
+    os.makedirs(d, exist_ok=True)
+    n = 1 << L
+    dx = 1.0
+    z = np.zeros((n, n))
+    occ = np.zeros((n, n), dtype=bool)
+    u = uniform(seed, 1 << 22)
+    k = 0
+    target = cover * n * n
+    covered = 0
+    while covered < target and k + 5 <= u.size:
+        w = 8 + int(u[k] * 33); hgt = 8 + int(u[k + 1] * 33)
+        x0 = int(u[k + 2] * (n - w)); y0 = int(u[k + 3] * (n - hgt))
+        zb = 5.0 + 15.0 * u[k + 4]
+        k += 5
+        blk = occ[y0:y0 + hgt, x0:x0 + w]
+        covered += int((~blk).sum())
+        blk[:] = True
+        z[y0:y0 + hgt, x0:x0 + w] = np.round(zb, 2)
+    strip = max(1, min(256, n // 16)) if n < 4096 else 256
+    h = np.zeros((n, n))
+    h[n - strip:, :] = 2.0
+    h[occ] = 0.0
+    write_raster(os.path.join(d, "case.dem"), z, 0.0, 0.0, dx, fmt="%.2f")
+    write_raster(os.path.join(d, "case.start"), h, 0.0, 0.0, dx, fmt="%.1f")
+    us = uniform(src_seed, 4096)
+    lines = []
+    j = 0
+    while len(lines) < n_src and j + 2 <= us.size:
+        xi = int(us[j] * n); yi = int(us[j + 1] * (n - strip)); j += 2
+        if not occ[yi, xi]:
+            lines.append(f"P {(xi + 0.5) * dx!r} {(yi + 0.5) * dx!r} QFIX {q_src / dx!r}")
+    with open(os.path.join(d, "case.bci"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    par = os.path.join(d, "case.par")
+    write_par(par, _par(solver, L, eps, t_end, manning, [("bcifile", "case.bci")]))
+    return par
+
+
+def random_case(d: str, L: int, seed: int, solver: str = "mwdg2", eps: float = 1e-3,
+                manning: float = 0.0, t_end: float = 1.0, wet_frac: float = 0.6,
+                bci: str | None = None, bdy: str | None = None, srcs: int = 0,
+                xsz: int | None = None, ysz: int | None = None, stage: int = 0,
+                refine_wall: int = 0, startq: bool = False) -> str:
+    """Small seeded case with wet and dry regions, for parity tests."""
+    os.makedirs(d, exist_ok=True)
+    n = 1 << L
+    xs = xsz or n
+    ys = ysz or n
+    dx = 1.0
+    z = gaussian_random_field(n, seed, 1.0)[:ys, :xs] + 2.0
+    eta0 = np.quantile(z, wet_frac)
+    h = np.maximum(0.0, eta0 - z)
+    bump = normal(seed + 1, ys * xs).reshape(ys, xs)
+    h = np.where(h > 0, h + 0.05 * np.abs(bump), 0.0)
+    write_raster(os.path.join(d, "case.dem"), z, 0.0, 0.0, dx)
+    write_raster(os.path.join(d, "case.start"), h, 0.0, 0.0, dx)
+    extra = []
+    if startq:
+        qx = 0.1 * normal(seed + 2, ys * xs).reshape(ys, xs) * (h > 0)
+        qy = 0.1 * normal(seed + 3, ys * xs).reshape(ys, xs) * (h > 0)
+        write_raster(os.path.join(d, "case.start.Qx"), qx, 0.0, 0.0, dx)
+        write_raster(os.path.join(d, "case.start.Qy"), qy, 0.0, 0.0, dx)
+        extra.append("startq2d")
+    lines = []
+    if bci:
+        lines.extend(bci.strip().splitlines())
+    if srcs:
+        us = uniform(seed + 7, 2 * srcs)
+        for s in range(srcs):
+            lines.append(f"P {int(us[2 * s] * xs) + 0.5!r} {int(us[2 * s + 1] * ys) + 0.5!r} QFIX 0.5")
+    if lines:
+        with open(os.path.join(d, "case.bci"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        extra.append(("bcifile", "case.bci"))
+    if bdy:
+        with open(os.path.join(d, "case.bdy"), "w") as f:
+            f.write(bdy)
+        extra.append(("bdyfile", "case.bdy"))
+    if stage:
+        us = uniform(seed + 9, 2 * stage)
+        with open(os.path.join(d, "case.stage"), "w") as f:
+            f.write(f"{stage}\n")
+            for s in range(stage):
+                f.write(f"{us[2 * s] * xs!r} {us[2 * s + 1] * ys!r}\n")
+        extra.append(("stagefile", "case.stage"))
+    if refine_wall:
+        extra.extend(["refine_wall", ("ref_thickness", refine_wall)])
+    par = os.path.join(d, "case.par")
+    write_par(par, _par(solver, L, eps, t_end, manning, extra))
+    return par

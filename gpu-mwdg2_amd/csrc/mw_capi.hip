@@ -1,0 +1,949 @@
+// Context, step orchestration and C ABI of the MI355X adaptive step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mwdg2.h"
+#include "mw_input.hpp"
+#include "mw_kernels.inc"
+
+namespace mw {
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define MW_HIP(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) throw ::mw::HipError(std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+inline int nblk(long long n, int b = WG) { return (int)std::max(1LL, (n + b - 1) / b); }
+
+template <typename T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  MW_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+// Host restatement of the series cursor used once at set-up
+// (PointSources constructor calls update_all_srcs(dt), PointSources.h:74).
+template <typename real>
+void advance_series_host(real t, int& row, const std::vector<real>& T, const std::vector<real>& V, real& out) {
+  const int n = (int)T.size();
+  if ((row - 1) < n && row + 1 < n) {
+    real t_1 = T[row], t_2 = T[row + 1];
+    if (t > t_2 && row + 2 < n) { row++; t_1 = T[row]; t_2 = T[row + 1]; }
+    else if (t > t_2) { out = V[n - 1]; return; }
+    const real i_1 = V[row], i_2 = V[row + 1];
+    out = i_1 + (i_2 - i_1) / (t_2 - t_1) * (t - t_1);
+  } else {
+    out = V[n - 1];
+  }
+}
+
+}  // namespace
+
+struct CtxBase {
+  virtual ~CtxBase() = default;
+  int prec = 8;
+  hipStream_t stream = nullptr;
+  bool profiling = false;
+  struct Prof { long long n = 0; double ms = 0; };
+  std::map<std::string, Prof> prof;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  hipEvent_t ev() {
+    if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
+    hipEvent_t e;
+    MW_HIP(hipEventCreate(&e));
+    return e;
+  }
+  template <typename F>
+  void timed(const char* name, F&& launch) {
+    if (!profiling) { launch(); return; }
+    hipEvent_t a = ev(), b = ev();
+    MW_HIP(hipEventRecord(a, stream));
+    launch();
+    MW_HIP(hipEventRecord(b, stream));
+    pending.push_back({name, {a, b}});
+  }
+  void drain_profile() {
+    for (auto& p : pending) {
+      MW_HIP(hipEventSynchronize(p.second.second));
+      float ms = 0;
+      MW_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+      Prof& q = prof[p.first];
+      q.n++;
+      q.ms += ms;
+      event_pool.push_back(p.second.first);
+      event_pool.push_back(p.second.second);
+    }
+    pending.clear();
+  }
+
+  virtual void step(int n) = 0;
+  virtual void counters(mw_counters* c) = 0;
+  virtual void stage(const std::string& s) = 0;
+  virtual long long transfer(const std::string& name, void* host, long long count, bool up) = 0;
+  virtual void set_scalar(const std::string& name, double v) = 0;
+  virtual void set_initial(const void* const* modal) = 0;
+  virtual void set_boundaries(const mw_boundary* b) = 0;
+  virtual void set_sources(int n, const mw_point_source* s) = 0;
+};
+
+template <typename real>
+struct Ctx : CtxBase {
+  HostCase<real> hc;
+  Params<real> P{};
+  int L = 0, N = 0, D = 0, H = 0, md = 0;
+  bool mwm = true;
+
+  DevState<real>* st = nullptr;
+  Hier<real> hier[2]{};        // MWDG2: [0] main (A), [1] stage buffer (B); HWFV1: ping-pong
+  int main_h = 0;
+  ZHier<real> zh{};
+  uint8_t* sig[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint8_t *xflag = nullptr, *leaflev = nullptr, *preflag = nullptr;
+  int* leaf = nullptr;
+  int* nbr[4] = {nullptr, nullptr, nullptr, nullptr};
+  int* tile_sums = nullptr;
+  int *rev_z = nullptr, *rev_rm = nullptr;
+  real* uprev = nullptr;
+  unsigned long long* maxh2 = nullptr;
+  // series and sources on the device
+  int *s_off = nullptr, *s_len = nullptr;
+  real *s_t = nullptr, *s_v = nullptr;
+  int *src_code = nullptr, *src_type = nullptr, *src_row = nullptr;
+  real* src_val = nullptr;
+  int n_src = 0;
+  bool issued_first = false;
+  std::vector<void*> allocs;
+
+  template <typename T>
+  T* alloc(size_t n) {
+    T* p = dalloc<T>(n);
+    allocs.push_back(p);
+    return p;
+  }
+
+  ~Ctx() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    for (auto& p : pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  Series<real> series() const { return {s_off, s_len, s_t, s_v}; }
+  Sources<real> sources() const { return {n_src, src_code, src_type, src_val, src_row}; }
+  LeafLists leaves() const { return {leaf, {nbr[0], nbr[1], nbr[2], nbr[3]}, &st->n_leaves}; }
+
+  void create(const mw_config& cfg) {
+    prec = sizeof(real);
+    read_case<real>(cfg.par_path, cfg.argc, cfg.argv, hc);
+    L = hc.L;
+    md = 1 << L;
+    N = md * md;
+    D = lvl_idx(L);
+    H = lvl_idx(L + 1);
+    mwm = hc.solver == MWDG2;
+    MW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const int nf = mwm ? 9 : 3;
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 9; ++v) {
+        const bool used = mwm || (v % 3 == 0);
+        hier[b].f[v] = used ? alloc<real>(H) : nullptr;
+        if (used) MW_HIP(hipMemsetAsync(hier[b].f[v], 0, sizeof(real) * H, stream));
+      }
+    (void)nf;
+    for (int m = 0; m < 3; ++m) {
+      zh.z[m] = (mwm || m == 0) ? alloc<real>(H) : nullptr;
+      if (zh.z[m]) MW_HIP(hipMemsetAsync(zh.z[m], 0, sizeof(real) * H, stream));
+    }
+    sig[0] = alloc<uint8_t>(D + 4);
+    sig[1] = alloc<uint8_t>(D + 4);
+    xflag = alloc<uint8_t>(D + 4);
+    leaflev = alloc<uint8_t>(D + 4);
+    preflag = alloc<uint8_t>(D + 4);
+    for (uint8_t* p : {sig[0], sig[1], xflag, leaflev, preflag}) MW_HIP(hipMemsetAsync(p, 0, D + 4, stream));
+    leaf = alloc<int>(N);
+    for (int k = 0; k < 4; ++k) nbr[k] = alloc<int>(N);
+    tile_sums = alloc<int>(nblk(std::max(1, N / 4), EMIT_TILE) + 1);
+    rev_z = alloc<int>(N);
+    rev_rm = alloc<int>(N);
+    st = alloc<DevState<real>>(1);
+    maxh2 = alloc<unsigned long long>(1);
+    if (hc.limitslopes && mwm) uprev = alloc<real>((size_t)9 * N);
+    fill_params();
+    upload_sources_and_series();
+    build_initial();
+  }
+
+  void fill_params() {
+    P.L = L; P.N = N; P.D = D; P.H = H; P.mesh_dim = md; P.xsz = hc.xsz; P.ysz = hc.ysz;
+    P.solver = hc.solver;
+    P.eps = hc.epsilon; P.dx = hc.dx; P.g = hc.g;
+    P.tol_h = real(1e-3); P.tol_q = real(0.0); P.tol_s = real(1e-9);   // classes/SolverParams.h:23-25
+    P.manning = hc.manning; P.initial_tstep = hc.initial_tstep; P.wall_height = hc.wall_height;
+    P.CFL = hc.CFL; P.tol_Krivo = hc.tol_Krivo;
+    P.eps_extra_factor = std::pow(real(2.0), real(1.5) + 1);          // extra_significance_kernel.cu:23
+    P.limitslopes = hc.limitslopes ? 1 : 0;
+    for (int k = 0; k < 4; ++k) { P.btype[k] = hc.bdy[k].type; P.bstart[k] = hc.bdy[k].start; P.bend[k] = hc.bdy[k].end; }
+  }
+
+  void upload_sources_and_series() {
+    n_src = (int)hc.srcs.size();
+    if (n_src > MAX_SRC) throw InputError("too many point sources");
+    std::vector<int> off, len;
+    std::vector<real> tt, vv;
+    auto add = [&](const HostSeries<real>& s) {
+      off.push_back((int)tt.size());
+      len.push_back((int)s.t.size());
+      tt.insert(tt.end(), s.t.begin(), s.t.end());
+      vv.insert(vv.end(), s.v.begin(), s.v.end());
+    };
+    for (int k = 0; k < 4; ++k) add(hc.bdy[k].series);
+    for (auto& s : hc.srcs) add(s.series);
+    s_off = alloc<int>(off.size());
+    s_len = alloc<int>(len.size());
+    s_t = alloc<real>(tt.size() + 1);
+    s_v = alloc<real>(vv.size() + 1);
+    MW_HIP(hipMemcpy(s_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    MW_HIP(hipMemcpy(s_len, len.data(), len.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!tt.empty()) {
+      MW_HIP(hipMemcpy(s_t, tt.data(), tt.size() * sizeof(real), hipMemcpyHostToDevice));
+      MW_HIP(hipMemcpy(s_v, vv.data(), vv.size() * sizeof(real), hipMemcpyHostToDevice));
+    }
+    std::vector<int> code(n_src + 1), type(n_src + 1), row(n_src + 1, 0);
+    std::vector<real> val(n_src + 1, real(0));
+    for (int s = 0; s < n_src; ++s) {
+      code[s] = hc.srcs[s].code;
+      type[s] = hc.srcs[s].type;
+      val[s] = hc.srcs[s].value;
+      if (type[s] == HVAR || type[s] == QVAR)             // PointSources.h:74, update_all_srcs(dt)
+        advance_series_host<real>(real(0.001), row[s], hc.srcs[s].series.t, hc.srcs[s].series.v, val[s]);
+    }
+    src_code = alloc<int>(n_src + 1);
+    src_type = alloc<int>(n_src + 1);
+    src_row = alloc<int>(n_src + 1);
+    src_val = alloc<real>(n_src + 1);
+    MW_HIP(hipMemcpy(src_code, code.data(), code.size() * sizeof(int), hipMemcpyHostToDevice));
+    MW_HIP(hipMemcpy(src_type, type.data(), type.size() * sizeof(int), hipMemcpyHostToDevice));
+    MW_HIP(hipMemcpy(src_row, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+    MW_HIP(hipMemcpy(src_val, val.data(), val.size() * sizeof(real), hipMemcpyHostToDevice));
+  }
+
+  // Radix sort of (key, value) pairs already on the device.
+  void sort_pairs(int n, int* k0, int* v0, int* k1, int* v1) {
+    const int tiles = nblk(n, RS_TILE);
+    int* hist = dalloc<int>((size_t)256 * tiles);
+    for (int pass = 0; pass < 4; ++pass) {
+      const int shift = 8 * pass;
+      hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, stream, k0, n, shift, hist, tiles);
+      hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, stream, hist, 256 * tiles);
+      hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(WG), 0, stream, k0, v0, k1, v1, n, shift, hist, tiles);
+      std::swap(k0, k1);
+      std::swap(v0, v1);
+    }
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(hist);
+    // after 4 passes the sorted data is back in the original buffers (k0/v0)
+  }
+
+  void zorder_build() {   // generate_all_morton_codes.cu:3-20 + get_sorting_indices.cu:3-62
+    int* codes = dalloc<int>(N);
+    int* idx = dalloc<int>(N);
+    int* k1 = dalloc<int>(N);
+    int* v1 = dalloc<int>(N);
+    hipLaunchKernelGGL(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);
+    sort_pairs(N, codes, idx, k1, v1);                    // idx now holds rev_z_order
+    MW_HIP(hipMemcpyAsync(rev_z, idx, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);  // idx = iota
+    int* keys = dalloc<int>(N);
+    MW_HIP(hipMemcpyAsync(keys, rev_z, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
+    sort_pairs(N, keys, idx, k1, v1);                     // idx now holds rev_row_major
+    MW_HIP(hipMemcpyAsync(rev_rm, idx, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    for (int* p : {codes, idx, k1, v1, keys}) (void)hipFree(p);
+  }
+
+  void build_initial() {
+    zorder_build();
+    const int modes = mwm ? 3 : 1;
+    // row-major modal values -> Z-ordered finest level
+    std::vector<real*> dm(12, nullptr);
+    for (int v = 0; v < 12; ++v)
+      if (!hc.modal[v].empty()) {
+        dm[v] = dalloc<real>(N);
+        MW_HIP(hipMemcpyAsync(dm[v], hc.modal[v].data(), sizeof(real) * N, hipMemcpyHostToDevice, stream));
+      } else {
+        dm[v] = nullptr;
+      }
+    real** dtab = dalloc<real*>(12);
+    MW_HIP(hipMemcpyAsync(dtab, dm.data(), sizeof(real*) * 12, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL((k_set_finest<real>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab, rev_z,
+                       hier[0], zh, L, N, modes);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipStreamSynchronize(stream));
+    for (real* p : dm) if (p) (void)hipFree(p);
+    (void)hipFree(dtab);
+
+    // preflags from gauges, boundaries, sources, walls (mra/preflag_details.cu:15-92)
+    std::vector<uint8_t> pf(D + 4, 0);
+    const int st0 = lvl_idx(L - 1);
+    for (int c : hc.stage_codes) pf[st0 + c / 4] = 1;
+    for (int k = 0; k < 4; ++k) for (int c : hc.bdy[k].codes) pf[st0 + c / 4] = 1;
+    for (auto& s : hc.srcs) pf[st0 + s.code / 4] = 1;
+    if (hc.refine_wall) {                                   // mra/refine_high_wall.cu:3-57
+      for (int j = 0; j < hc.ref_thickness; ++j)
+        for (int i = 0; i < hc.xsz; ++i) { const int y = hc.ysz - 1 - j; if (y >= 0) pf[st0 + morton(i, y) / 4] = 1; }
+      for (int j = 0; j < hc.ref_thickness; ++j)
+        for (int i = 0; i < hc.ysz; ++i) { const int x = hc.xsz - 1 - j; if (x >= 0) pf[st0 + morton(x, i) / 4] = 1; }
+    }
+    MW_HIP(hipMemcpy(preflag, pf.data(), D + 4, hipMemcpyHostToDevice));
+
+    // maxima of the initial assembled solution, including the point-source
+    // insertion into it (mra/insert_point_srcs.cu:3-29, get_max_scale_coeffs.cu)
+    real mx[4] = {0, 0, 0, 0};
+    {
+      const std::vector<real>& h0 = hc.modal[0];
+      const std::vector<real>& qx0 = hc.modal[3];
+      const std::vector<real>& z0 = hc.modal[9];
+      std::map<int, real> hmod;   // Z index -> modified h0
+      for (int s = 0; s < n_src; ++s) {
+        const int code = hc.srcs[s].code;
+        const int rm = compact(code >> 1) * md + compact(code);
+        const real cur_h = hmod.count(code) ? hmod[code] : h0[rm];
+        real val = hc.srcs[s].value;
+        int row = 0;
+        if (hc.srcs[s].type == HVAR || hc.srcs[s].type == QVAR)
+          advance_series_host<real>(real(0.001), row, hc.srcs[s].series.t, hc.srcs[s].series.v, val);
+        if (hc.srcs[s].type == HFIX || hc.srcs[s].type == HVAR) {
+          const int rs = compact(s >> 1) * md + compact(s);   // z0 at Z index s (quirk)
+          hmod[code] = val - z0[rs];
+        } else {
+          hmod[code] = cur_h + val * real(0.001) / hc.dx;
+        }
+      }
+      std::vector<int> rm_of_mod;
+      for (size_t i = 0; i < (size_t)N; ++i) {
+        const real h = h0[i];
+        const real e = rabs<real>(h + z0[i]);
+        mx[0] = e > mx[0] ? e : mx[0];
+        const real ah = rabs<real>(h);
+        mx[1] = ah > mx[1] ? ah : mx[1];
+        const real aq = rabs<real>(qx0[i]);
+        mx[2] = aq > mx[2] ? aq : mx[2];
+        const real az = rabs<real>(z0[i]);
+        mx[3] = az > mx[3] ? az : mx[3];
+      }
+      // the maxima must be those of the modified values: rescan if a source
+      // lowered a cell that held a maximum
+      if (!hmod.empty()) {
+        std::vector<char> is_mod(N, 0);
+        for (auto& kv : hmod) is_mod[compact(kv.first >> 1) * md + compact(kv.first)] = 1;
+        real m0 = 0, m1 = 0;
+        for (size_t i = 0; i < (size_t)N; ++i) {
+          real h = h0[i];
+          if (is_mod[i]) {
+            const int code = morton((int)(i % md), (int)(i / md));
+            h = hmod[code];
+          }
+          const real e = rabs<real>(h + z0[i]);
+          m0 = e > m0 ? e : m0;
+          const real ah = rabs<real>(h);
+          m1 = ah > m1 ? ah : m1;
+        }
+        mx[0] = m0;
+        mx[1] = m1;
+      }
+    }
+    real mxs[5];
+    mxs[0] = rmax<real>(real(1.0), mx[0]);
+    mxs[1] = rmax<real>(real(1.0), mx[1]);
+    mxs[2] = rmax<real>(real(1.0), mx[2]);
+    mxs[3] = mxs[2];
+    mxs[4] = rmax<real>(real(1.0), mx[3]);
+
+    // z hierarchy and topography preflags (mra/preflag_topo.cu:3-43)
+    for (int level = L - 1; level >= 0; --level) {
+      if (mwm)
+        hipLaunchKernelGGL((k_z_encode<real, true>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+                           level, L, hc.epsilon, mxs[4]);
+      else
+        hipLaunchKernelGGL((k_z_encode<real, false>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+                           level, L, hc.epsilon, mxs[4]);
+    }
+    MW_HIP(hipGetLastError());
+    // all parents significant before the first encode (mra/init_sig_details.cu)
+    cur = 0;
+    MW_HIP(hipMemsetAsync(sig[cur], 1, D + 4, stream));
+    DevState<real> s{};
+    s.time = real(0);
+    s.dt = real(0.001);                                      // run_simulation.cu:37
+    s.sim_time = hc.sim_time;
+    for (int k = 0; k < 5; ++k) s.mx[k] = mxs[k];
+    s.dt_acc = 0;
+    s.n_leaves = N;
+    s.done = 0;
+    s.first = 1;
+    s.steps = 0;
+    for (int k = 0; k < 4; ++k) { s.inlet[k] = hc.bdy[k].inlet; s.row[k] = 0; }
+    s.n_src = n_src;
+    MW_HIP(hipMemcpyAsync(st, &s, sizeof s, hipMemcpyHostToDevice, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    main_h = 0;
+    issued_first = false;
+  }
+
+  // ---- step pieces ---------------------------------------------------------
+  void k_prologue_launch() {
+    timed("prologue", [&] { hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(1), 0, stream, st, P, series(), sources()); });
+  }
+  void k_sources_launch() {
+    if (!n_src) return;
+    timed("point_sources", [&] {
+      hipLaunchKernelGGL((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx);
+    });
+  }
+  void encode_launch() {
+    uint8_t* prev = sig[cur];
+    uint8_t* nw = sig[cur ^ 1];
+    for (int level = L - 1; level >= 0; --level) {
+      const long long n = 1LL << (2 * level);
+      timed("encode", [&] {
+        if (mwm)
+          hipLaunchKernelGGL((k_encode<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                             preflag, level, L, hc.epsilon, P.eps_extra_factor);
+        else
+          hipLaunchKernelGGL((k_encode<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                             preflag, level, L, hc.epsilon, P.eps_extra_factor);
+      });
+    }
+  }
+  void decode_launch() {
+    uint8_t* prev = sig[cur];
+    uint8_t* nw = sig[cur ^ 1];
+    for (int level = 0; level < L; ++level) {
+      const long long n = 1LL << (2 * level);
+      timed("decode", [&] {
+        if (mwm)
+          hipLaunchKernelGGL((k_topdown<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                             leaflev, level, L, hc.epsilon, P.eps_extra_factor);
+        else
+          hipLaunchKernelGGL((k_topdown<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                             leaflev, level, L, hc.epsilon, P.eps_extra_factor);
+      });
+    }
+  }
+  void emit_launch() {
+    const int Q = 1 << (2 * (L - 1));
+    const int tiles = nblk(Q, EMIT_TILE);
+    int* done = &st->done;
+    int* nl = &st->n_leaves;
+    timed("emit_count", [&] { hipLaunchKernelGGL(k_emit_count, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums); });
+    timed("emit_scan", [&] { hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles); });
+    timed("emit_write", [&] {
+      hipLaunchKernelGGL(k_emit_write, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums, leaf, nbr[0],
+                         nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
+    });
+  }
+  int leaf_grid() const { return std::min(nblk(N), 256 * 16); }
+  void stage1_launch() {
+    const int other = main_h ^ 1;
+    timed(mwm ? "stage1" : "fv1", [&] {
+      if (mwm) {
+        if (hc.limitslopes)
+          hipLaunchKernelGGL((k_stage1<real, true, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
+                             hier[main_h], hier[other], zh, uprev);
+        else
+          hipLaunchKernelGGL((k_stage1<real, true, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
+                             hier[main_h], hier[other], zh, uprev);
+      } else {
+        hipLaunchKernelGGL((k_stage1<real, false, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
+                           hier[main_h], hier[other], zh, uprev);
+      }
+    });
+    if (!mwm) main_h = other;
+  }
+  void encode_scale_launch() {
+    for (int level = L - 1; level >= 0; --level) {
+      const long long n = 1LL << (2 * level);
+      timed("encode_scale", [&] {
+        hipLaunchKernelGGL((k_encode_scale<real>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[1], sig[cur], level);
+      });
+    }
+  }
+  void stage2_launch() {
+    if (hc.limitslopes) {
+      MW_HIP(hipMemsetAsync(maxh2, 0, sizeof(unsigned long long), stream));
+      timed("maxh2", [&] {
+        hipLaunchKernelGGL((k_maxh_stage2<real>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[1], zh, maxh2);
+      });
+      timed("stage2", [&] {
+        hipLaunchKernelGGL((k_stage2<real, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
+                           zh, uprev, maxh2);
+      });
+    } else {
+      timed("stage2", [&] {
+        hipLaunchKernelGGL((k_stage2<real, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0],
+                           hier[1], zh, uprev, maxh2);
+      });
+    }
+  }
+
+  void rebuild() {
+    encode_launch();
+    decode_launch();
+    emit_launch();
+    cur ^= 1;
+  }
+
+  void one_step() {
+    k_prologue_launch();
+    k_sources_launch();
+    if (hc.epsilon > real(0.0) || !issued_first) rebuild();
+    stage1_launch();
+    if (mwm) {
+      if (hc.epsilon > real(0.0)) encode_scale_launch();
+      stage2_launch();
+    }
+    issued_first = true;
+  }
+
+  void step(int n) override {
+    for (int i = 0; i < n; ++i) one_step();
+    MW_HIP(hipGetLastError());
+    if (profiling) drain_profile();
+  }
+
+  DevState<real> read_state() {
+    DevState<real> s;
+    MW_HIP(hipMemcpyAsync(&s, st, sizeof s, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    return s;
+  }
+
+  void counters(mw_counters* c) override {
+    const DevState<real> s = read_state();
+    c->time = s.time;
+    c->dt = s.dt;
+    c->dt_next = s.first ? (double)s.dt : (double)Bits<real>::from(s.dt_acc);
+    c->sim_time = s.sim_time;
+    c->steps = s.steps;
+    c->n_leaves = s.n_leaves;
+    c->done = s.done;
+    c->max_ref_lvl = L;
+    c->solver = hc.solver;
+    c->precision = (int)sizeof(real);
+    for (int k = 0; k < 5; ++k) c->maxes[k] = s.mx[k];
+  }
+
+  void stage(const std::string& s) override {
+    if (s == "prologue") k_prologue_launch();
+    else if (s == "point_sources") k_sources_launch();
+    else if (s == "encode") encode_launch();
+    else if (s == "decode") decode_launch();
+    else if (s == "emit") { emit_launch(); }
+    else if (s == "swap_masks") cur ^= 1;
+    else if (s == "stage1" || s == "fv1") stage1_launch();
+    else if (s == "encode_scale") encode_scale_launch();
+    else if (s == "stage2") stage2_launch();
+    else if (s == "mark_first_issued") issued_first = true;
+    else throw std::invalid_argument("unknown stage " + s);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipStreamSynchronize(stream));
+    if (profiling) drain_profile();
+  }
+
+  long long transfer(const std::string& name, void* host, long long count, bool up) override {
+    void* dev = nullptr;
+    long long len = 0;
+    size_t esz = 0;
+    const char* fn[9] = {"eta0", "eta1x", "eta1y", "qx0", "qx1x", "qx1y", "qy0", "qy1x", "qy1y"};
+    if (name.size() > 2 && (name[0] == 'A' || name[0] == 'B') && name[1] == '.') {
+      const int b = (name[0] == 'A') ? (mwm ? 0 : main_h) : (mwm ? 1 : main_h ^ 1);
+      for (int v = 0; v < 9; ++v)
+        if (name.substr(2) == fn[v]) { dev = hier[b].f[v]; len = H; esz = sizeof(real); }
+    } else if (name == "z0" || name == "z1x" || name == "z1y") {
+      dev = zh.z[name == "z0" ? 0 : (name == "z1x" ? 1 : 2)]; len = H; esz = sizeof(real);
+    } else if (name == "sig") { dev = sig[cur]; len = D; esz = 1; }
+    else if (name == "sig_next") { dev = sig[cur ^ 1]; len = D; esz = 1; }
+    else if (name == "xflag") { dev = xflag; len = D; esz = 1; }
+    else if (name == "leaflev") { dev = leaflev; len = D; esz = 1; }
+    else if (name == "preflag") { dev = preflag; len = D; esz = 1; }
+    else if (name == "leaves" || name.rfind("nbr", 0) == 0) {
+      const DevState<real> s = read_state();
+      len = s.n_leaves;
+      esz = sizeof(int);
+      dev = (name == "leaves") ? (void*)leaf : (void*)nbr[name[3] - '0'];
+    } else if (name == "rev_z_order") { dev = rev_z; len = N; esz = sizeof(int); }
+    else if (name == "rev_row_major") { dev = rev_rm; len = N; esz = sizeof(int); }
+    if (!dev) throw std::invalid_argument("unknown or absent array " + name);
+    if (host && count > 0) {
+      const size_t bytes = (size_t)std::min(count, len) * esz;
+      if (up) MW_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
+      else MW_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+      MW_HIP(hipStreamSynchronize(stream));
+    }
+    return len;
+  }
+
+  void set_scalar(const std::string& name, double v) override {
+    DevState<real> s = read_state();
+    if (name == "time") s.time = (real)v;
+    else if (name == "dt") s.dt = (real)v;
+    else if (name == "dt_next") s.dt_acc = Bits<real>::to_host((real)v);
+    else if (name == "n_leaves") s.n_leaves = (int)v;
+    else if (name == "first") s.first = (int)v;
+    else if (name == "mx_eta") s.mx[0] = (real)v;
+    else if (name == "mx_h") s.mx[1] = (real)v;
+    else if (name == "mx_qx") s.mx[2] = (real)v;
+    else if (name == "mx_qy") s.mx[3] = (real)v;
+    else if (name == "mx_z") s.mx[4] = (real)v;
+    else if (name == "issued_first") { issued_first = v != 0; return; }
+    else if (name == "cur") { cur = (int)v & 1; return; }
+    else throw std::invalid_argument("unknown scalar " + name);
+    MW_HIP(hipMemcpyAsync(st, &s, sizeof s, hipMemcpyHostToDevice, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+  }
+
+  void set_initial(const void* const* modal) override {
+    for (int v = 0; v < 12; ++v) {
+      const bool used = mwm || (v % 3 == 0);
+      if (!used) continue;
+      if (!modal[v]) throw std::invalid_argument("missing modal array");
+      hc.modal[v].assign((const real*)modal[v], (const real*)modal[v] + N);
+    }
+    rebuild_all();
+  }
+  void set_boundaries(const mw_boundary* b) override {
+    for (int k = 0; k < 4; ++k) {
+      HostBoundary<real>& d = hc.bdy[k];
+      d = HostBoundary<real>();
+      d.type = b[k].type; d.start = b[k].start; d.end = b[k].end; d.inlet = (real)b[k].inlet;
+      if ((d.type == HVAR || d.type == QVAR) && (b[k].n_series < 1 || !b[k].series_t || !b[k].series_v))
+        throw std::invalid_argument("HVAR/QVAR boundary needs a time series");
+      if (d.type != CLOSED && d.type != FREE && d.type != HFIX && d.type != HVAR && d.type != QFIX && d.type != QVAR)
+        throw std::invalid_argument("unknown boundary type");
+      for (int i = 0; i < b[k].n_series; ++i) { d.series.t.push_back((real)b[k].series_t[i]); d.series.v.push_back((real)b[k].series_v[i]); }
+      for (int i = 0; i < d.end - d.start + 1; ++i) {
+        const int c = d.start + i;
+        const int x = (k == 0 || k == 2) ? c : (k == 1 ? hc.xsz - 1 : 0);
+        const int y = (k == 0) ? hc.ysz - 1 : (k == 2 ? 0 : c);
+        if (x < 0 || y < 0 || x >= md || y >= md) throw InputError("boundary span leaves the mesh");
+        d.codes.push_back(morton(x, y));
+      }
+    }
+    rebuild_all();
+  }
+  void set_sources(int n, const mw_point_source* s) override {
+    hc.srcs.clear();
+    for (int i = 0; i < n; ++i) {
+      if (s[i].x < 0 || s[i].y < 0 || s[i].x >= md || s[i].y >= md) throw InputError("point source outside the mesh");
+      HostSource<real> d;
+      d.code = morton(s[i].x, s[i].y); d.type = s[i].type; d.value = (real)s[i].value;
+      if ((d.type == HVAR || d.type == QVAR) && (s[i].n_series < 1 || !s[i].series_t || !s[i].series_v))
+        throw std::invalid_argument("HVAR/QVAR point source needs a time series");
+      if (d.type != HFIX && d.type != HVAR && d.type != QFIX && d.type != QVAR)
+        throw std::invalid_argument("unknown point source type");
+      for (int j = 0; j < s[i].n_series; ++j) { d.series.t.push_back((real)s[i].series_t[j]); d.series.v.push_back((real)s[i].series_v[j]); }
+      hc.srcs.push_back(d);
+    }
+    rebuild_all();
+  }
+  void rebuild_all() {
+    MW_HIP(hipStreamSynchronize(stream));
+    for (int b = 0; b < 2; ++b)
+      for (int v = 0; v < 9; ++v)
+        if (hier[b].f[v]) MW_HIP(hipMemsetAsync(hier[b].f[v], 0, sizeof(real) * H, stream));
+    for (int m = 0; m < 3; ++m) if (zh.z[m]) MW_HIP(hipMemsetAsync(zh.z[m], 0, sizeof(real) * H, stream));
+    for (uint8_t* p : {sig[0], sig[1], xflag, leaflev, preflag}) MW_HIP(hipMemsetAsync(p, 0, D + 4, stream));
+    fill_params();
+    upload_sources_and_series();
+    build_initial();
+  }
+};
+
+}  // namespace mw
+
+// ============================================================================
+// C ABI
+// ============================================================================
+struct mw_ctx {
+  std::unique_ptr<mw::CtxBase> impl;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return MW_OK;
+  } catch (const mw::InputError& e) {
+    mw::g_err = e.what();
+    return MW_ERR_INPUT;
+  } catch (const mw::HipError& e) {
+    mw::g_err = e.what();
+    return MW_ERR_HIP;
+  } catch (const std::invalid_argument& e) {
+    mw::g_err = e.what();
+    return MW_ERR_ARG;
+  } catch (const std::exception& e) {
+    mw::g_err = e.what();
+    return MW_ERR_STATE;
+  }
+}
+
+#define MW_NEED(c) \
+  if (!(c)) { mw::g_err = "invalid argument: " #c; return MW_ERR_ARG; }
+
+}  // namespace
+
+extern "C" {
+
+const char* mw_version(void) { return "mwdg2-mi355x 0.1"; }
+const char* mw_last_error(void) { return mw::g_err.c_str(); }
+
+int mw_create(const mw_config* cfg, mw_ctx** out) {
+  MW_NEED(cfg && out && cfg->par_path);
+  MW_NEED(cfg->precision == 4 || cfg->precision == 8);
+  *out = nullptr;
+  return guarded([&] {
+    auto c = std::make_unique<mw_ctx>();
+    if (cfg->precision == 4) {
+      auto p = std::make_unique<mw::Ctx<float>>();
+      p->create(*cfg);
+      c->impl = std::move(p);
+    } else {
+      auto p = std::make_unique<mw::Ctx<double>>();
+      p->create(*cfg);
+      c->impl = std::move(p);
+    }
+    *out = c.release();
+  });
+}
+
+int mw_destroy(mw_ctx* ctx) {
+  delete ctx;
+  return MW_OK;
+}
+
+int mw_set_initial(mw_ctx* ctx, const void* const* modal) {
+  MW_NEED(ctx && modal);
+  return guarded([&] { ctx->impl->set_initial(modal); });
+}
+int mw_set_boundaries(mw_ctx* ctx, const mw_boundary* b4) {
+  MW_NEED(ctx && b4);
+  return guarded([&] { ctx->impl->set_boundaries(b4); });
+}
+int mw_set_point_sources(mw_ctx* ctx, int n, const mw_point_source* src) {
+  MW_NEED(ctx && n >= 0 && (n == 0 || src));
+  return guarded([&] { ctx->impl->set_sources(n, src); });
+}
+
+int mw_step(mw_ctx* ctx, int n) {
+  MW_NEED(ctx && n >= 0);
+  return guarded([&] { ctx->impl->step(n); });
+}
+
+int mw_sync(mw_ctx* ctx) {
+  MW_NEED(ctx);
+  return guarded([&] {
+    hipError_t e = hipStreamSynchronize(ctx->impl->stream);
+    if (e != hipSuccess) throw mw::HipError(hipGetErrorString(e));
+  });
+}
+
+int mw_counters_get(mw_ctx* ctx, mw_counters* out) {
+  MW_NEED(ctx && out);
+  return guarded([&] { ctx->impl->counters(out); });
+}
+
+int mw_run_until(mw_ctx* ctx, long long max_steps, int batch, mw_counters* out) {
+  MW_NEED(ctx && batch > 0);
+  mw_counters c{};
+  int rc = MW_OK;
+  long long taken = 0;
+  while (true) {
+    rc = mw_counters_get(ctx, &c);
+    if (rc != MW_OK) return rc;
+    if (c.done || (max_steps >= 0 && c.steps >= max_steps)) break;
+    if (c.steps > 0 && !(c.dt_next > 0.0)) {
+      mw::g_err = "time step collapsed to " + std::to_string(c.dt_next) + " at t = " + std::to_string(c.time);
+      if (out) *out = c;
+      return MW_ERR_NUMERIC;
+    }
+    int n = batch;
+    if (max_steps >= 0) n = (int)std::min<long long>(batch, max_steps - c.steps);
+    rc = mw_step(ctx, n);
+    if (rc != MW_OK) return rc;
+    taken += n;
+  }
+  if (out) *out = c;
+  return MW_OK;
+}
+
+static int stage(mw_ctx* ctx, const char* s) {
+  MW_NEED(ctx);
+  return guarded([&] { ctx->impl->stage(s); });
+}
+int mw_zorder_build(mw_ctx* ctx) {
+  MW_NEED(ctx);
+  return guarded([&] {
+    if (auto* f = dynamic_cast<mw::Ctx<float>*>(ctx->impl.get())) f->zorder_build();
+    else dynamic_cast<mw::Ctx<double>*>(ctx->impl.get())->zorder_build();
+  });
+}
+int mw_prologue(mw_ctx* ctx) { return stage(ctx, "prologue"); }
+int mw_point_sources(mw_ctx* ctx) { return stage(ctx, "point_sources"); }
+int mw_encode(mw_ctx* ctx) { return stage(ctx, "encode"); }
+int mw_decode(mw_ctx* ctx) { return stage(ctx, "decode"); }
+int mw_extract_leaves(mw_ctx* ctx) { return stage(ctx, "emit"); }
+int mw_update_stage(mw_ctx* ctx, int s) {
+  MW_NEED(s >= 0 && s <= 2);
+  return stage(ctx, s == 0 ? "fv1" : (s == 1 ? "stage1" : "stage2"));
+}
+int mw_encode_scale(mw_ctx* ctx) { return stage(ctx, "encode_scale"); }
+int mw_cfl_min(mw_ctx* ctx, double* dt_next) {
+  MW_NEED(ctx && dt_next);
+  mw_counters c{};
+  const int rc = mw_counters_get(ctx, &c);
+  if (rc == MW_OK) *dt_next = c.dt_next;
+  return rc;
+}
+int mw_stage_named(mw_ctx* ctx, const char* name) {
+  MW_NEED(name);
+  return stage(ctx, name);
+}
+
+int mw_download(mw_ctx* ctx, const char* name, void* host, long long count, long long* len) {
+  MW_NEED(ctx && name);
+  return guarded([&] {
+    const long long l = ctx->impl->transfer(name, host, count, false);
+    if (len) *len = l;
+  });
+}
+int mw_upload(mw_ctx* ctx, const char* name, const void* host, long long count) {
+  MW_NEED(ctx && name && host);
+  return guarded([&] { ctx->impl->transfer(name, const_cast<void*>(host), count, true); });
+}
+int mw_set_scalar(mw_ctx* ctx, const char* name, double v) {
+  MW_NEED(ctx && name);
+  return guarded([&] { ctx->impl->set_scalar(name, v); });
+}
+
+int mw_sort_pairs(int n, const int* keys, const int* vals, int* keys_out, int* vals_out) {
+  MW_NEED(n >= 0 && (n == 0 || (keys && vals && keys_out && vals_out)));
+  if (n == 0) return MW_OK;
+  return guarded([&] {
+    mw::Ctx<float> tmp;
+    MW_HIP(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+    int* k0 = tmp.alloc<int>(n);
+    int* v0 = tmp.alloc<int>(n);
+    int* k1 = tmp.alloc<int>(n);
+    int* v1 = tmp.alloc<int>(n);
+    MW_HIP(hipMemcpy(k0, keys, sizeof(int) * n, hipMemcpyHostToDevice));
+    MW_HIP(hipMemcpy(v0, vals, sizeof(int) * n, hipMemcpyHostToDevice));
+    tmp.sort_pairs(n, k0, v0, k1, v1);
+    MW_HIP(hipMemcpy(keys_out, k0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    MW_HIP(hipMemcpy(vals_out, v0, sizeof(int) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int mw_profile(mw_ctx* ctx, int enable) {
+  MW_NEED(ctx);
+  return guarded([&] {
+    ctx->impl->drain_profile();
+    ctx->impl->profiling = enable != 0;
+    if (enable) ctx->impl->prof.clear();
+  });
+}
+
+int mw_profile_read(mw_ctx* ctx, mw_kernel_time* out, int max_entries, int* n_entries) {
+  MW_NEED(ctx && n_entries);
+  return guarded([&] {
+    ctx->impl->drain_profile();
+    int i = 0;
+    for (auto& kv : ctx->impl->prof) {
+      if (out && i < max_entries) {
+        std::snprintf(out[i].name, sizeof out[i].name, "%s", kv.first.c_str());
+        out[i].launches = kv.second.n;
+        out[i].total_ms = kv.second.ms;
+      }
+      ++i;
+    }
+    *n_entries = i;
+  });
+}
+
+int mw_filter_table(int precision, void* out144) {
+  MW_NEED(out144 && (precision == 4 || precision == 8));
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const int o = ((f * 4 + k) * 3 + i) * 3 + j;
+          if (precision == 8) static_cast<double*>(out144)[o] = mw::FB[f][k][i][j].d;
+          else static_cast<float*>(out144)[o] = mw::FB[f][k][i][j].f;
+        }
+  return MW_OK;
+}
+
+// Drop-in for the reference binary (MAIN.cu:16-22, run_simulation.cu:3-800).
+int mw_run_simulation(int argc, const char* const* argv) {
+  MW_NEED(argc >= 2 && argv);
+  const char* par = argv[argc - 1];
+  const bool f32 = std::getenv("MW_PRECISION") && std::string(std::getenv("MW_PRECISION")) == "4";
+  mw_config cfg{par, f32 ? 4 : 8, argc, argv};
+  mw_ctx* ctx = nullptr;
+  int rc = mw_create(&cfg, &ctx);
+  if (rc != MW_OK) { std::fprintf(stderr, "mwdg2: %s\n", mw_last_error()); return rc; }
+  mw_counters c{};
+  const auto t0 = std::chrono::steady_clock::now();
+  mw_counters_get(ctx, &c);
+  double n = (double)(1 << c.max_ref_lvl);
+  long long last_print = -1;
+  while (true) {
+    rc = mw_run_until(ctx, c.steps + 10000, 64, &c);
+    if (rc != MW_OK) break;
+    if (c.steps / 10000 != last_print) {            // run_simulation.cu:753-762
+      last_print = c.steps / 10000;
+      std::printf("Elements: %lld, compression: %f%%, time step: %f, timestep: %lld, sim time: %f\n", c.n_leaves,
+                  100.0 - 100.0 * c.n_leaves / (n * n), c.dt_next, c.steps, c.time);
+    }
+    if (c.done) break;
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (rc == MW_OK) {
+    std::printf("Elements: %lld, compression: %f%%, time step: %f, timestep: %lld, sim time: %f\n", c.n_leaves,
+                100.0 - 100.0 * c.n_leaves / (n * n), c.dt_next, c.steps + 1, c.time);
+    std::printf("Loop time: %f s\n", secs);
+  } else {
+    std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
+  }
+  mw_destroy(ctx);
+  return rc;
+}
+
+}  // extern "C"

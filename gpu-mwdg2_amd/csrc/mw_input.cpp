@@ -1,0 +1,365 @@
+// Host reader of the drop-in input files. See mw_input.hpp.
+#include "mw_input.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace mw {
+
+namespace {
+
+template <typename real> const char* num_fmt();
+template <> const char* num_fmt<float>() { return "%f"; }
+template <> const char* num_fmt<double>() { return "%lf"; }
+
+int zmorton(int x, int y) {
+  auto spread = [](int v) {
+    v &= 0xffff;
+    v = (v ^ (v << 8)) & 0x00ff00ff;
+    v = (v ^ (v << 4)) & 0x0f0f0f0f;
+    v = (v ^ (v << 2)) & 0x33333333;
+    v = (v ^ (v << 1)) & 0x55555555;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+
+struct File {
+  FILE* fp;
+  explicit File(const std::string& name, bool required = true) : fp(std::fopen(name.c_str(), "r")) {
+    if (!fp && required) throw InputError("cannot open " + name);
+  }
+  ~File() { if (fp) std::fclose(fp); }
+  File(const File&) = delete;
+  File& operator=(const File&) = delete;
+};
+
+// Scan lines until the first token starts with kw; returns the line.
+bool find_line(const char* file, const char* kw, char (&line)[256]) {
+  File f(file);
+  const size_t n = std::strlen(kw);
+  char tok[128] = {0};
+  while (std::fgets(line, sizeof line, f.fp)) {
+    char t[128] = {0};
+    if (std::sscanf(line, "%127s", t) == 1) std::memcpy(tok, t, sizeof tok);
+    if (std::strncmp(tok, kw, n) == 0) return true;
+  }
+  return false;
+}
+
+// The reference opens names relative to the working directory; a name that
+// is not found there is looked up next to the .par file.
+std::string beside(const char* par, const std::string& name) {
+  if (name.empty()) return name;
+  if (FILE* fp = std::fopen(name.c_str(), "r")) { std::fclose(fp); return name; }
+  const std::string p(par);
+  const size_t s = p.find_last_of('/');
+  return s == std::string::npos ? name : p.substr(0, s + 1) + name;
+}
+
+template <typename real>
+void read_raster(const std::string& name, std::vector<real>& out, int mesh_dim, real wall) {
+  File f(name, false);                // input/read_raster_file.cpp:18-24: missing -> defaults
+  if (!f.fp) return;
+  char buf[256];
+  int ncols = 0, nrows = 0;
+  real dummy = 0;
+  if (std::fscanf(f.fp, "%255s %d", buf, &ncols) != 2 || std::fscanf(f.fp, "%255s %d", buf, &nrows) != 2)
+    throw InputError("bad raster header in " + name);
+  if (ncols > mesh_dim || nrows > mesh_dim || ncols < 0 || nrows < 0)
+    throw InputError("raster " + name + " is larger than the 2^max_ref_lvl mesh");
+  char fmt[24];
+  std::snprintf(fmt, sizeof fmt, "%%255s %s", num_fmt<real>());
+  for (int k = 0; k < 4; ++k) std::fscanf(f.fp, fmt, buf, &dummy);
+  for (int j = 0; j < nrows; ++j)
+    for (int i = 0; i < ncols; ++i) {
+      if (std::fscanf(f.fp, num_fmt<real>(), &dummy) != 1) throw InputError("short raster " + name);
+      const bool nodata = std::fabs(dummy + real(9999.0)) < real(1e-10);
+      out[(size_t)(nrows - 1 - j) * mesh_dim + i] = nodata ? wall : dummy;
+    }
+}
+
+template <typename real>
+void read_series(const char* par, const char* name, HostSeries<real>& s) {
+  const std::string bdy = beside(par, par_str(par, "bdyfile"));
+  if (bdy.empty()) throw InputError("time series '" + std::string(name) + "' needs a bdyfile");
+  File f(bdy);
+  char line[256], tok[128] = {0};
+  const size_t n = std::strlen(name);
+  bool found = false;
+  while (std::fgets(line, sizeof line, f.fp)) {
+    char t[128] = {0};
+    if (std::sscanf(line, "%127s", t) == 1) std::memcpy(tok, t, sizeof tok);
+    if (std::strncmp(tok, name, n) == 0) { found = true; break; }
+  }
+  if (!found) throw InputError("time series '" + std::string(name) + "' not in " + bdy);
+  int rows = 0;
+  char unit[64] = {0};
+  if (!std::fgets(line, sizeof line, f.fp) || std::sscanf(line, "%d %63s", &rows, unit) < 1 || rows <= 0)
+    throw InputError("zero-row time series '" + std::string(name) + "'");
+  const int mult = !std::strncmp(unit, "seconds", 7) ? 1 : (!std::strncmp(unit, "minutes", 7) ? 60 : 3600);
+  char fmt[16];
+  std::snprintf(fmt, sizeof fmt, "%s %s", num_fmt<real>(), num_fmt<real>());
+  for (int i = 0; i < rows; ++i) {
+    real v = 0, t = 0;
+    if (!std::fgets(line, sizeof line, f.fp)) throw InputError("short time series '" + std::string(name) + "'");
+    std::sscanf(line, fmt, &v, &t);
+    s.v.push_back(v);
+    s.t.push_back(t * mult);
+  }
+}
+
+int type_of(const char* t) {
+  if (!std::strncmp(t, "CLOSED", 6)) return 1;
+  if (!std::strncmp(t, "FREE", 4)) return 2;
+  if (!std::strncmp(t, "HFIX", 4)) return 4;
+  if (!std::strncmp(t, "HVAR", 4)) return 8;
+  if (!std::strncmp(t, "QFIX", 4)) return 16;
+  if (!std::strncmp(t, "QVAR", 4)) return 32;
+  return 0;
+}
+
+}  // namespace
+
+bool par_has(const char* file, const char* kw) {
+  char line[256];
+  return find_line(file, kw, line);
+}
+
+int par_int(const char* file, const char* kw) {
+  File f(file);
+  const size_t n = std::strlen(kw);
+  char line[256], tok[128] = {0};
+  int v = 0;
+  while (std::fgets(line, sizeof line, f.fp)) {
+    char t[128] = {0};
+    int tv = v;
+    const int got = std::sscanf(line, "%127s %d", t, &tv);
+    if (got >= 1) std::memcpy(tok, t, sizeof tok);
+    if (got == 2) v = tv;
+    if (std::strncmp(tok, kw, n) == 0) return v;
+  }
+  return 0;
+}
+
+template <typename real>
+real par_real(const char* file, const char* kw) {
+  File f(file);
+  const size_t n = std::strlen(kw);
+  char line[256], tok[128] = {0}, fmt[24];
+  std::snprintf(fmt, sizeof fmt, "%%127s %s", num_fmt<real>());
+  real v = 0;
+  while (std::fgets(line, sizeof line, f.fp)) {
+    char t[128] = {0};
+    real tv = v;
+    const int got = std::sscanf(line, fmt, t, &tv);
+    if (got >= 1) std::memcpy(tok, t, sizeof tok);
+    if (got == 2) v = tv;
+    if (std::strncmp(tok, kw, n) == 0) return v;
+  }
+  return real(0);
+}
+
+std::string par_str(const char* file, const char* kw) {
+  File f(file);
+  const size_t n = std::strlen(kw);
+  char line[256], tok[128] = {0}, val[256] = {0};
+  while (std::fgets(line, sizeof line, f.fp)) {
+    char t[128] = {0}, tv[256] = {0};
+    const int got = std::sscanf(line, "%127s %255s", t, tv);
+    if (got >= 1) std::memcpy(tok, t, sizeof tok);
+    if (got == 2) std::memcpy(val, tv, sizeof val);
+    if (std::strncmp(tok, kw, n) == 0) return std::string(val);
+  }
+  return std::string();
+}
+
+template <typename real>
+void read_case(const char* par, int argc, const char* const* argv, HostCase<real>& c) {
+  {
+    File probe(par);
+  }
+  if (par_int(par, "test_case") != 0)                       // input/read_test_case.cpp:3-17
+    throw InputError("only raster-driven cases (test_case 0) are supported");
+  c.L = par_int(par, "max_ref_lvl");
+  if (c.L < 2 || c.L > 14) throw InputError("max_ref_lvl must be in [2, 14]");
+  c.initial_tstep = par_real<real>(par, "initial_tstep");
+  c.epsilon = par_real<real>(par, "epsilon");
+  c.wall_height = par_real<real>(par, "wall_height");
+  if (par_has(par, "hwfv1")) { c.solver = 1; c.CFL = real(0.5); }
+  else if (par_has(par, "mwdg2")) { c.solver = 2; c.CFL = real(0.3); }
+  else throw InputError("invalid adaptive solver type: the .par needs a line 'hwfv1' or 'mwdg2'");
+  c.limitslopes = par_has(par, "limitslopes");
+  if (c.limitslopes) c.tol_Krivo = par_real<real>(par, "tol_Krivo");
+  c.refine_wall = par_has(par, "refine_wall");
+  if (c.refine_wall) c.ref_thickness = par_int(par, "ref_thickness");
+  c.startq2d = par_has(par, "startq2d");
+  if (par_has(par, "cfl")) c.CFL = par_real<real>(par, "cfl");    // extension keyword
+
+  const std::string dem = beside(par, par_str(par, "DEMfile"));
+  if (dem.empty()) throw InputError("the .par names no DEMfile");
+  c.xsz = par_int(dem.c_str(), "ncols");
+  c.ysz = par_int(dem.c_str(), "nrows");
+  c.xmin = par_real<real>(dem.c_str(), "xllcorner");
+  c.ymin = par_real<real>(dem.c_str(), "yllcorner");
+  c.dx = par_real<real>(dem.c_str(), "cellsize");
+  c.sim_time = par_real<real>(par, "sim_time");
+  c.manning = par_real<real>(par, "fpfric");
+  c.monai = par_has(par, "monai");
+  c.oregon = par_has(par, "oregon-seaside");
+  if (c.monai || c.oregon) throw InputError("the Monai and Oregon wavemaker boundaries are not supported");
+
+  for (int i = 0; i + 1 < argc; ++i) {                       // input/read_command_line_params.cpp:12-47
+    if (!std::strncmp(argv[i], "-epsilon", 8)) {
+      real v = 0;
+      std::sscanf(argv[i + 1], num_fmt<real>(), &v);
+      c.epsilon = v;
+    } else if (!std::strncmp(argv[i], "-solver", 7)) {
+      if (!std::strncmp(argv[i + 1], "hwfv1", 5)) { c.solver = 1; c.CFL = real(0.5); }
+      else if (!std::strncmp(argv[i + 1], "mwdg2", 5)) { c.solver = 2; c.CFL = real(0.3); }
+      else throw InputError("invalid -solver value");
+    }
+  }
+  const int md = 1 << c.L;
+  if (c.xsz < 1 || c.ysz < 1 || c.xsz > md || c.ysz > md) throw InputError("DEM size does not fit the mesh");
+  if (!(c.dx > real(0))) throw InputError("DEM cellsize must be positive");
+
+  // boundaries (classes/Boundary.h:237-369)
+  const std::string bci_name = par_str(par, "bcifile");
+  const std::string bci = beside(par, bci_name);
+  const char dchar[4] = {'N', 'E', 'S', 'W'};
+  if (!bci_name.empty()) {
+    for (int k = 0; k < 4; ++k) {
+      File f(bci);
+      char line[256] = {0}, t[16] = {0}, series[64] = {0}, dir = 0;
+      real lower = 0, upper = 0, inlet = 0;
+      char fmt4[40], fmt5[48];
+      std::snprintf(fmt4, sizeof fmt4, "%%c %s %s %%15s", num_fmt<real>(), num_fmt<real>());
+      bool found = false;
+      while (std::fgets(line, sizeof line, f.fp)) {
+        std::sscanf(line, fmt4, &dir, &lower, &upper, t);
+        if (dir == dchar[k]) { found = true; break; }
+      }
+      if (!found) continue;
+      HostBoundary<real>& b = c.bdy[k];
+      b.type = type_of(t);
+      if (b.type == 0) b.type = 1;
+      if (b.type == 4 || b.type == 16) {
+        std::snprintf(fmt5, sizeof fmt5, "%%c %s %s %%15s %s", num_fmt<real>(), num_fmt<real>(), num_fmt<real>());
+        std::sscanf(line, fmt5, &dir, &lower, &upper, t, &inlet);
+      } else if (b.type == 8 || b.type == 32) {
+        std::snprintf(fmt5, sizeof fmt5, "%%c %s %s %%15s %%63s", num_fmt<real>(), num_fmt<real>());
+        std::sscanf(line, fmt5, &dir, &lower, &upper, t, series);
+        read_series<real>(par, series, b.series);
+      }
+      const real origin = (k == 0 || k == 2) ? c.xmin : c.ymin;
+      b.start = (int)((lower - origin) / c.dx);
+      b.end = (int)((upper - origin) / c.dx - 1);
+      b.inlet = inlet;
+      for (int i = 0; i < b.end - b.start + 1; ++i) {          // Boundary.h:182-235
+        const int cc = b.start + i;
+        const int x = (k == 2) ? cc : (k == 0) ? cc : (k == 1) ? c.xsz - 1 : 0;
+        const int y = (k == 2) ? 0 : (k == 0) ? c.ysz - 1 : cc;
+        if (x < 0 || y < 0 || x >= md || y >= md) throw InputError("boundary span leaves the mesh");
+        b.codes.push_back(zmorton(x, y));
+      }
+    }
+    // point sources (classes/PointSources.h:188-286)
+    File f(bci);
+    char line[256] = {0};
+    while (std::fgets(line, sizeof line, f.fp)) {
+      char t[16] = {0}, series[64] = {0}, pt = 0;
+      real xs = 0, ys = 0, v = 0;
+      char fmt4[40], fmt5[48];
+      std::snprintf(fmt4, sizeof fmt4, "%%c %s %s %%15s", num_fmt<real>(), num_fmt<real>());
+      std::sscanf(line, fmt4, &pt, &xs, &ys, t);
+      if (pt != 'P') continue;
+      HostSource<real> s;
+      const int x = (int)((xs - c.xmin) / c.dx), y = (int)((ys - c.ymin) / c.dx);
+      if (x < 0 || y < 0 || x >= md || y >= md) throw InputError("point source outside the mesh");
+      s.code = zmorton(x, y);
+      s.type = type_of(t);
+      if (s.type == 4 || s.type == 16) {
+        std::snprintf(fmt5, sizeof fmt5, "%%c %s %s %%15s %s", num_fmt<real>(), num_fmt<real>(), num_fmt<real>());
+        std::sscanf(line, fmt5, &pt, &xs, &ys, t, &v);
+        s.value = v;
+      } else if (s.type == 8 || s.type == 32) {
+        std::snprintf(fmt5, sizeof fmt5, "%%c %s %s %%15s %%63s", num_fmt<real>(), num_fmt<real>());
+        std::sscanf(line, fmt5, &pt, &xs, &ys, t, series);
+        read_series<real>(par, series, s.series);
+      } else {
+        throw InputError("point source with an unknown type");
+      }
+      c.srcs.push_back(s);
+    }
+  }
+  // gauge points, used for preflagging (classes/StagePoints.h:40-140)
+  if (par_has(par, "stagefile")) {
+    const std::string sf = beside(par, par_str(par, "stagefile"));
+    File f(sf);
+    int n = 0;
+    if (std::fscanf(f.fp, "%d", &n) != 1) n = 0;
+    char fmt2[16];
+    std::snprintf(fmt2, sizeof fmt2, "%s %s", num_fmt<real>(), num_fmt<real>());
+    for (int i = 0; i < n; ++i) {
+      real xs = 0, ys = 0;
+      if (std::fscanf(f.fp, fmt2, &xs, &ys) != 2) throw InputError("short stage file");
+      const int x = (int)((xs - c.xmin) / c.dx), y = (int)((ys - c.ymin) / c.dx);
+      if (x < 0 || y < 0 || x >= md || y >= md) throw InputError("stage point outside the mesh");
+      c.stage_codes.push_back(zmorton(x, y));
+    }
+  }
+
+  // initial modal state (mra/get_modal_values.cu:29-52)
+  const size_t N = (size_t)md * md;
+  std::vector<real> hr(N, real(0)), qxr(N, real(0)), qyr(N, real(0)), zr(N, c.wall_height);
+  const std::string start = beside(par, par_str(par, "startfile"));
+  if (!start.empty()) read_raster<real>(start, hr, md, c.wall_height);
+  read_raster<real>(dem, zr, md, c.wall_height);
+  if (c.startq2d && !start.empty()) {
+    read_raster<real>(start + ".Qx", qxr, md, c.wall_height);
+    read_raster<real>(start + ".Qy", qyr, md, c.wall_height);
+  }
+  for (auto& a : c.modal) a.clear();
+  if (c.solver == 1) {                                          // input/read_and_project_modes_fv1.cu:39-53
+    c.modal[0] = std::move(hr);
+    c.modal[3] = std::move(qxr);
+    c.modal[6] = std::move(qyr);
+    c.modal[9] = std::move(zr);
+    return;
+  }
+  // nodes from cells with the reference's flips and duplicated last row/column
+  // (input/read_and_project_modes_dg2.cu:51-80), then 4-node averages and
+  // slopes (mra/modal_projections.cu:71-86)
+  const int nd = md + 1, iy = c.ysz + 1, ix = c.xsz + 1;
+  std::vector<real> node[4];
+  for (auto& a : node) a.assign((size_t)nd * nd, real(0));
+  const std::vector<real>* src[4] = {&hr, &qxr, &qyr, &zr};
+  for (int y = 0; y < nd; ++y)
+    for (int x = 0; x < nd; ++x) {
+      const int i = (x == ix - 1 || x == nd - 1) ? x - 1 : x;
+      const int j = (y == iy - 1 || y == nd - 1) ? y - 1 : y;
+      const size_t ii = (size_t)(nd - 1 - y) * nd + x, mi = (size_t)(md - 1 - j) * md + i;
+      for (int v = 0; v < 4; ++v) node[v][ii] = (*src[v])[mi];
+    }
+  for (auto& a : c.modal) a.assign(N, real(0));
+  const real s3 = std::sqrt(real(3.0));
+  for (size_t idx = 0; idx < N; ++idx) {
+    const size_t x = idx % md, y = idx / md;
+    const size_t SW = y * nd + x, SE = SW + 1, NW = (y + 1) * nd + x, NE = NW + 1;
+    for (int v = 0; v < 4; ++v) {
+      const std::vector<real>& a = node[v];
+      c.modal[3 * v][idx] = real(0.25) * (a[NW] + a[NE] + a[SE] + a[SW]);
+      c.modal[3 * v + 1][idx] = (a[NE] - a[NW] + a[SE] - a[SW]) / (real(4.0) * s3);
+      c.modal[3 * v + 2][idx] = (a[NE] - a[SE] + a[NW] - a[SW]) / (real(4.0) * s3);
+    }
+  }
+}
+
+template void read_case<float>(const char*, int, const char* const*, HostCase<float>&);
+template void read_case<double>(const char*, int, const char* const*, HostCase<double>&);
+template float par_real<float>(const char*, const char*);
+template double par_real<double>(const char*, const char*);
+
+}  // namespace mw

@@ -1,0 +1,196 @@
+"""Python binding of the C ABI in include/mwdg2.h (ctypes over libmwdg2.so).
+
+The library is the product: HIP kernels for gfx950 plus the host step loop.
+Loading fails loudly when the built library is missing; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmwdg2.so")
+
+MW_OK = 0
+ERRORS = {1: "input", 2: "hip", 3: "argument", 4: "state", 5: "numeric"}
+
+
+class MWError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mwdg2 {ERRORS.get(code, code)} error: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("par_path", ctypes.c_char_p), ("precision", ctypes.c_int), ("argc", ctypes.c_int),
+                ("argv", ctypes.POINTER(ctypes.c_char_p))]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("sim_time", ctypes.c_double), ("steps", ctypes.c_longlong), ("n_leaves", ctypes.c_longlong),
+                ("done", ctypes.c_int), ("max_ref_lvl", ctypes.c_int), ("solver", ctypes.c_int),
+                ("precision", ctypes.c_int), ("maxes", ctypes.c_double * 5)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "maxes"}
+        d["maxes"] = list(self.maxes)
+        return d
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MWError(4, f"{LIB_PATH} is not built; run gpu-mwdg2_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, cp, i, d, ll = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+        L.mw_last_error.restype = cp
+        L.mw_version.restype = cp
+        L.mw_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        L.mw_destroy.argtypes = [vp]
+        L.mw_step.argtypes = [vp, i]
+        L.mw_sync.argtypes = [vp]
+        L.mw_run_until.argtypes = [vp, ll, i, ctypes.POINTER(Counters)]
+        L.mw_counters_get.argtypes = [vp, ctypes.POINTER(Counters)]
+        for n in ("mw_zorder_build", "mw_prologue", "mw_point_sources", "mw_encode", "mw_decode",
+                  "mw_extract_leaves", "mw_encode_scale"):
+            getattr(L, n).argtypes = [vp]
+        L.mw_update_stage.argtypes = [vp, i]
+        L.mw_stage_named.argtypes = [vp, cp]
+        L.mw_cfl_min.argtypes = [vp, ctypes.POINTER(d)]
+        L.mw_download.argtypes = [vp, cp, vp, ll, ctypes.POINTER(ll)]
+        L.mw_upload.argtypes = [vp, cp, vp, ll]
+        L.mw_set_scalar.argtypes = [vp, cp, d]
+        L.mw_set_initial.argtypes = [vp, ctypes.POINTER(vp)]
+        L.mw_sort_pairs.argtypes = [i, vp, vp, vp, vp]
+        L.mw_profile.argtypes = [vp, i]
+        L.mw_profile_read.argtypes = [vp, ctypes.POINTER(KernelTime), i, ctypes.POINTER(i)]
+        L.mw_filter_table.argtypes = [i, vp]
+        L.mw_run_simulation.argtypes = [i, ctypes.POINTER(cp)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != MW_OK:
+        raise MWError(rc, lib().mw_last_error().decode())
+
+
+_BYTE = ("sig", "sig_next", "xflag", "leaflev", "preflag")
+_INT = ("leaves", "nbr0", "nbr1", "nbr2", "nbr3", "rev_z_order", "rev_row_major")
+
+
+class Simulation:
+    """One scenario on the GPU, created from a .par file (the drop-in input)."""
+
+    def __init__(self, par: str, precision: str = "f32", argv=()):
+        self.prec = 4 if precision in ("f32", 4, "float32") else 8
+        self.real = np.float32 if self.prec == 4 else np.float64
+        self._argv = (ctypes.c_char_p * max(1, len(argv)))(*[a.encode() for a in argv])
+        cfg = Config(par.encode(), self.prec, len(argv), self._argv)
+        h = ctypes.c_void_p()
+        _check(lib().mw_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # stepping ------------------------------------------------------------
+    def step(self, n: int = 1):
+        _check(lib().mw_step(self._h, n))
+
+    def sync(self):
+        _check(lib().mw_sync(self._h))
+
+    def run(self, max_steps: int = -1, batch: int = 32) -> dict:
+        c = Counters()
+        _check(lib().mw_run_until(self._h, max_steps, batch, ctypes.byref(c)))
+        return c.as_dict()
+
+    def counters(self) -> dict:
+        c = Counters()
+        _check(lib().mw_counters_get(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def stage(self, name: str):
+        _check(lib().mw_stage_named(self._h, name.encode()))
+
+    # data ----------------------------------------------------------------
+    def _dtype(self, name):
+        if name in _BYTE:
+            return np.uint8
+        if name in _INT:
+            return np.int32
+        return self.real
+
+    def get(self, name: str) -> np.ndarray:
+        n = ctypes.c_longlong()
+        _check(lib().mw_download(self._h, name.encode(), None, 0, ctypes.byref(n)))
+        a = np.empty(n.value, dtype=self._dtype(name))
+        if n.value:
+            _check(lib().mw_download(self._h, name.encode(), a.ctypes.data, n.value, ctypes.byref(n)))
+        return a
+
+    def set(self, name: str, values):
+        a = np.ascontiguousarray(values, dtype=self._dtype(name))
+        _check(lib().mw_upload(self._h, name.encode(), a.ctypes.data, a.size))
+
+    def set_scalar(self, name: str, v: float):
+        _check(lib().mw_set_scalar(self._h, name.encode(), float(v)))
+
+    def set_initial(self, modal):
+        arrs = [np.ascontiguousarray(m, dtype=self.real) for m in modal]
+        ptrs = (ctypes.c_void_p * 12)(*[a.ctypes.data for a in arrs])
+        _check(lib().mw_set_initial(self._h, ptrs))
+
+    # profiling -----------------------------------------------------------
+    def profile(self, enable: bool = True):
+        _check(lib().mw_profile(self._h, int(enable)))
+
+    def profile_read(self) -> dict:
+        n = ctypes.c_int()
+        _check(lib().mw_profile_read(self._h, None, 0, ctypes.byref(n)))
+        buf = (KernelTime * max(1, n.value))()
+        _check(lib().mw_profile_read(self._h, buf, n.value, ctypes.byref(n)))
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(n.value)}
+
+
+def sort_pairs(keys, vals):
+    k = np.ascontiguousarray(keys, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.int32)
+    ko = np.empty_like(k)
+    vo = np.empty_like(v)
+    _check(lib().mw_sort_pairs(k.size, k.ctypes.data, v.ctypes.data, ko.ctypes.data, vo.ctypes.data))
+    return ko, vo
+
+
+def filter_table(precision: int = 8) -> np.ndarray:
+    out = np.zeros(144, dtype=np.float64 if precision == 8 else np.float32)
+    _check(lib().mw_filter_table(precision, out.ctypes.data))
+    return out.reshape(4, 4, 3, 3)
+
+
+def run_simulation(argv) -> int:
+    a = (ctypes.c_char_p * len(argv))(*[x.encode() for x in argv])
+    return lib().mw_run_simulation(len(argv), a)

@@ -1,0 +1,148 @@
+/*
+ * C ABI of the MI355X-native adaptive MWDG2/HWFV1 shallow-water step.
+ *
+ * The reference (al0vya/gpu-mwdg2) has no FFI: its boundary is the C++ call
+ * surface between the step loop of run_simulation (source/run_simulation.cu:
+ * 226-774) and the kernel wrappers it calls, fed by .par/raster/.bci/.bdy
+ * files. Each entry point below names the reference routine(s) it replaces.
+ * All arguments are plain pointers, sizes and numbers; device work runs on one
+ * HIP stream owned by the context. Every call returns MW_OK or an error code
+ * and never exits the process; mw_last_error() describes the last failure of
+ * the calling thread. One context may be driven by one host thread at a time.
+ */
+#ifndef MWDG2_H
+#define MWDG2_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MW_OK 0
+#define MW_ERR_INPUT 1     /* bad or missing input file / parameter        */
+#define MW_ERR_HIP 2       /* HIP runtime failure (allocation, launch, ...) */
+#define MW_ERR_ARG 3       /* invalid argument (null, size, unknown name)   */
+#define MW_ERR_STATE 4     /* call not valid in the context's state         */
+#define MW_ERR_NUMERIC 5   /* time step collapsed to <= 0 or became NaN     */
+
+typedef struct mw_ctx mw_ctx;
+
+typedef struct mw_config {
+  const char* par_path;          /* .par file (run_simulation.cu:11)               */
+  int precision;                 /* 4 = FP32, 8 = FP64 (types/real.h:3-19)         */
+  int argc;                      /* command-line overrides -epsilon/-solver        */
+  const char* const* argv;       /* (input/read_command_line_params.cpp:3-48)      */
+} mw_config;
+
+typedef struct mw_counters {
+  double time;                   /* simulated time reached                         */
+  double dt;                     /* time step of the last step taken               */
+  double dt_next;                /* CFL step reduced by the last step (get_dt_CFL) */
+  double sim_time;               /* end time (sim_time keyword)                    */
+  long long steps;               /* steps taken                                    */
+  long long n_leaves;            /* active leaves N_a (d_assem_sol.length)         */
+  int done;                      /* 1 once time >= sim_time                        */
+  int max_ref_lvl;
+  int solver;                    /* 1 = HWFV1, 2 = MWDG2                           */
+  int precision;
+  double maxes[5];               /* eta, h, qx, qy, z used by the last encode      */
+} mw_counters;
+
+typedef struct mw_boundary {     /* classes/Boundary.h:17-31, one side            */
+  int type;                      /* 1 CLOSED, 2 FREE, 4 HFIX, 8 HVAR, 16 QFIX, 32 QVAR */
+  int start, end;                /* cell span along the side (inclusive)          */
+  double inlet;                  /* HFIX level or QFIX unit discharge             */
+  int n_series;                  /* HVAR/QVAR: knots (time, value)                */
+  const double* series_t;
+  const double* series_v;
+} mw_boundary;
+
+typedef struct mw_point_source { /* classes/PointSources.h:15-29, one source     */
+  int x, y;                      /* finest cell                                    */
+  int type;                      /* 4 HFIX, 8 HVAR, 16 QFIX, 32 QVAR               */
+  double value;
+  int n_series;
+  const double* series_t;
+  const double* series_v;
+} mw_point_source;
+
+typedef struct mw_kernel_time {
+  char name[32];
+  long long launches;
+  double total_ms;
+} mw_kernel_time;
+
+const char* mw_version(void);
+const char* mw_last_error(void);
+
+/* Set-up: reads the .par and the files it names, builds the Z-ordered finest
+ * level, the z hierarchy and the preflags on the GPU.
+ * Replaces run_simulation.cu:9-223 (SolverParams, SimulationParams,
+ * Boundaries, PointSources, get_modal_values, generate_all_morton_codes,
+ * get_sorting_indices, sort_finest_scale_coeffs_z_order,
+ * copy_finest_coefficients, insert_point_srcs, preflag_details, preflag_topo). */
+int mw_create(const mw_config* cfg, mw_ctx** out);
+int mw_destroy(mw_ctx* ctx);
+
+/* Replace the initial state with row-major finest modal values, 12 arrays of
+ * 4^L values in the context's precision: h0 h1x h1y qx0 qx1x qx1y qy0 qy1x
+ * qy1y z0 z1x z1y (HWFV1 reads the *0 arrays only). Rebuilds what mw_create
+ * derived from the rasters (get_modal_values.cu:3-54 onwards). */
+int mw_set_initial(mw_ctx* ctx, const void* const* modal);
+/* Replace the four boundaries N,E,S,W (Boundaries, classes/Boundaries.h:5-40). */
+int mw_set_boundaries(mw_ctx* ctx, const mw_boundary* b4);
+/* Replace the point sources (classes/PointSources.h:31-75). */
+int mw_set_point_sources(mw_ctx* ctx, int n, const mw_point_source* src);
+
+/* Enqueue n whole time steps (one pass of run_simulation.cu:226-774 each);
+ * asynchronous, no host round trip. Steps after the end time do nothing. */
+int mw_step(mw_ctx* ctx, int n);
+/* Run until time >= sim_time or max_steps, checking every `batch` steps. */
+int mw_run_until(mw_ctx* ctx, long long max_steps, int batch, mw_counters* out);
+int mw_sync(mw_ctx* ctx);
+int mw_counters_get(mw_ctx* ctx, mw_counters* out);
+
+/* Stage-level entry points (parity and profiling). */
+int mw_zorder_build(mw_ctx* ctx);        /* generate_all_morton_codes.cu + get_sorting_indices.cu */
+int mw_prologue(mw_ctx* ctx);            /* run_simulation.cu:228-235,247,260,479                */
+int mw_point_sources(mw_ctx* ctx);       /* mra/reinsert_point_srcs.cu                           */
+int mw_encode(mw_ctx* ctx);              /* encode_flow(for_nghbrs=false) + regularisation        */
+int mw_decode(mw_ctx* ctx);              /* extra_significance + decoding (+ leaf labels)         */
+int mw_extract_leaves(mw_ctx* ctx);      /* traverse + rev_z_order_act_idcs + find_neighbours +
+                                            get_compaction_flags + sort_neighbours_z_order +
+                                            compaction                                           */
+int mw_update_stage(mw_ctx* ctx, int stage); /* 1: load_soln+ghosts+friction+(limiter)+dg2 stage 1;
+                                                2: load_soln+ghosts+(limiter)+dg2 stage 2 + get_dt_CFL;
+                                                0: HWFV1 load_soln+ghosts+friction+fv1_update+get_dt_CFL */
+int mw_encode_scale(mw_ctx* ctx);        /* reinsert + encode_flow(for_nghbrs=true)               */
+int mw_cfl_min(mw_ctx* ctx, double* dt_next); /* operators/get_dt_CFL.cu:3-50 (reads the device min) */
+
+/* Named device arrays: hierarchy "A.eta0".."A.qy1y", "B.*", "z0","z1x","z1y";
+ * masks "sig" (current tree), "sig_next", "xflag", "leaflev", "preflag";
+ * leaf lists "leaves", "nbr0".."nbr3" (N,E,S,W); "rev_z_order","rev_row_major".
+ * count is in elements; *len (optional) receives the array length. */
+int mw_download(mw_ctx* ctx, const char* name, void* host, long long count, long long* len);
+int mw_upload(mw_ctx* ctx, const char* name, const void* host, long long count);
+/* Scalars: "time","dt","dt_next","n_leaves","first","mx_eta","mx_h","mx_qx","mx_qy","mx_z". */
+int mw_set_scalar(mw_ctx* ctx, const char* name, double v);
+
+/* Stable LSD radix sort of n (key, value) int pairs held in host memory,
+ * done on the GPU (cub::DeviceRadixSort::SortPairs, get_sorting_indices.cu:24-57). */
+int mw_sort_pairs(int n, const int* keys, const int* vals, int* keys_out, int* vals_out);
+
+/* Per-kernel timing with HIP events on the context stream. */
+int mw_profile(mw_ctx* ctx, int enable);
+int mw_profile_read(mw_ctx* ctx, mw_kernel_time* out, int max_entries, int* n_entries);
+
+/* Filter bank literals [family HH,GA,GB,GC][k][i][j] as used by the kernels
+ * (mra/Filters.h:202-397); 144 doubles (precision 8) or floats (precision 4). */
+int mw_filter_table(int precision, void* out144);
+
+/* Drop-in for the reference executable: argv[argc-1] is the .par file
+ * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */
+int mw_run_simulation(int argc, const char* const* argv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MWDG2_H */

@@ -1,0 +1,337 @@
+// C API of the CPU oracle, for the parity tests and the CPU-baseline timing.
+// TEST INFRASTRUCTURE ONLY (see mw_oracle.hpp). One translation unit.
+#include "mw_oracle.hpp"
+
+#include <memory>
+
+#include "mw_oracle_io.inc"
+#include "mw_oracle_mra.inc"
+#include "mw_oracle_ops.inc"
+
+namespace mwo {
+template struct Oracle<float>;
+template struct Oracle<double>;
+}  // namespace mwo
+
+using namespace mwo;
+
+namespace {
+
+struct Handle {
+  int prec = 8;
+  std::unique_ptr<Oracle<float>> f;
+  std::unique_ptr<Oracle<double>> d;
+};
+
+template <typename real>
+std::vector<real>* real_array(Oracle<real>& o, const std::string& n) {
+  std::vector<real>* hier[12] = {&o.eta0, &o.eta1x, &o.eta1y, &o.qx0, &o.qx1x, &o.qx1y,
+                                 &o.qy0, &o.qy1x, &o.qy1y, &o.z0, &o.z1x, &o.z1y};
+  const char* hn[12] = {"eta0", "eta1x", "eta1y", "qx0", "qx1x", "qx1y", "qy0", "qy1x", "qy1y", "z0", "z1x", "z1y"};
+  for (int i = 0; i < 12; ++i)
+    if (n == hn[i]) return hier[i];
+  if (n == "norm") return &o.norm;
+  if (n == "dt_cfl") return &o.dt_cfl;
+  // det_<var>_<mode>_<family>, var in eta,qx,qy,z; mode 0,1x,1y; family a,b,g
+  if (n.rfind("det_", 0) == 0) {
+    const char* vars[4] = {"eta", "qx", "qy", "z"};
+    const char* modes[3] = {"0", "1x", "1y"};
+    const char* fams[3] = {"a", "b", "g"};
+    for (int v = 0; v < 4; ++v)
+      for (int m = 0; m < 3; ++m)
+        for (int a = 0; a < 3; ++a)
+          if (n == std::string("det_") + vars[v] + "_" + modes[m] + "_" + fams[a]) return &o.det[v][m][a];
+  }
+  // <set>.<field>, set in assem, buf, nb0..nb3
+  const size_t dot = n.find('.');
+  if (dot != std::string::npos) {
+    const std::string set = n.substr(0, dot), fld = n.substr(dot + 1);
+    Sol<real>* s = nullptr;
+    if (set == "assem") s = &o.assem;
+    else if (set == "buf") s = &o.buf;
+    else if (set.size() == 3 && set.rfind("nb", 0) == 0) s = &o.nb[set[2] - '0'];
+    if (s) {
+      std::vector<real>* f[12] = {&s->h0, &s->h1x, &s->h1y, &s->qx0, &s->qx1x, &s->qx1y,
+                                  &s->qy0, &s->qy1x, &s->qy1y, &s->z0, &s->z1x, &s->z1y};
+      const char* fn[12] = {"h0", "h1x", "h1y", "qx0", "qx1x", "qx1y", "qy0", "qy1x", "qy1y", "z0", "z1x", "z1y"};
+      for (int i = 0; i < 12; ++i)
+        if (fld == fn[i]) return f[i];
+    }
+  }
+  return nullptr;
+}
+
+template <typename real>
+std::vector<int>* int_array(Oracle<real>& o, const std::string& n) {
+  if (n == "rev_z_order") return &o.rev_z_order;
+  if (n == "rev_row_major") return &o.rev_row_major;
+  if (n == "morton") return &o.morton_codes;
+  const size_t dot = n.find('.');
+  if (dot != std::string::npos) {
+    const std::string set = n.substr(0, dot), fld = n.substr(dot + 1);
+    Sol<real>* s = nullptr;
+    if (set == "assem") s = &o.assem;
+    else if (set == "buf") s = &o.buf;
+    else if (set.size() == 3 && set.rfind("nb", 0) == 0) s = &o.nb[set[2] - '0'];
+    else if (set.size() == 6 && set.rfind("bufnb", 0) == 0) s = &o.bufnb[set[5] - '0'];
+    if (s && fld == "act") return &s->act;
+    if (s && fld == "lev") return &s->lev;
+  }
+  return nullptr;
+}
+
+template <typename real>
+std::vector<uint8_t>* byte_array(Oracle<real>& o, const std::string& n) {
+  if (n == "sig") return &o.sig;
+  if (n == "preflag") return &o.preflag;
+  if (n == "flag_ne") return &o.flag_ne;
+  if (n == "flag_sw") return &o.flag_sw;
+  return nullptr;
+}
+
+template <typename real>
+long get_set(Oracle<real>& o, const char* name, void* buf, long n, bool set) {
+  const std::string s(name);
+  auto io = [&](auto* vec) -> long {
+    if (!vec) return -1;
+    using T = typename std::remove_pointer<decltype(vec)>::type::value_type;
+    const long len = (long)vec->size();
+    if (buf) {
+      const long m = std::min(n, len);
+      if (set) std::memcpy(vec->data(), buf, m * sizeof(T));
+      else std::memcpy(buf, vec->data(), m * sizeof(T));
+    }
+    return len;
+  };
+  long r;
+  if ((r = io(real_array(o, s))) >= 0) return r;
+  if ((r = io(int_array(o, s))) >= 0) return r;
+  if ((r = io(byte_array(o, s))) >= 0) return r;
+  return -1;
+}
+
+template <typename real>
+double scalar(Oracle<real>& o, const std::string& n) {
+  if (n == "dt") return o.dt;
+  if (n == "time") return o.current_time;
+  if (n == "sim_time") return o.p.time;
+  if (n == "L") return o.p.L;
+  if (n == "N") return o.N;
+  if (n == "D") return o.D;
+  if (n == "H") return o.H;
+  if (n == "length") return o.assem.length;
+  if (n == "buf_length") return o.buf.length;
+  if (n == "first") return o.first;
+  if (n == "timestep") return o.timestep;
+  if (n == "mx_eta") return o.mx_eta;
+  if (n == "mx_h") return o.mx_h;
+  if (n == "mx_qx") return o.mx_qx;
+  if (n == "mx_qy") return o.mx_qy;
+  if (n == "mx_z") return o.mx_z;
+  if (n == "epsilon") return o.p.epsilon;
+  if (n == "dx") return o.p.dx;
+  if (n == "solver") return o.p.solver;
+  if (n == "CFL") return o.p.CFL;
+  if (n == "xsz") return o.p.xsz;
+  if (n == "ysz") return o.p.ysz;
+  if (n == "P_e") return o.last_P_e;
+  if (n == "P_d") return o.last_P_d;
+  if (n == "manning") return o.p.manning;
+  if (n == "initial_tstep") return o.p.initial_tstep;
+  if (n == "inlet_N") return o.bN.inlet;
+  if (n == "inlet_E") return o.bE.inlet;
+  if (n == "inlet_S") return o.bS.inlet;
+  if (n == "inlet_W") return o.bW.inlet;
+  return std::nan("");
+}
+
+template <typename real>
+int set_scalar(Oracle<real>& o, const std::string& n, double v) {
+  if (n == "dt") o.dt = (real)v;
+  else if (n == "time") o.current_time = (real)v;
+  else if (n == "length") o.assem.length = (int)v;
+  else if (n == "buf_length") o.buf.length = (int)v;
+  else if (n == "first") o.first = v != 0;
+  else if (n == "mx_eta") o.mx_eta = (real)v;
+  else if (n == "mx_h") o.mx_h = (real)v;
+  else if (n == "mx_qx") o.mx_qx = (real)v;
+  else if (n == "mx_qy") o.mx_qy = (real)v;
+  else if (n == "mx_z") o.mx_z = (real)v;
+  else return -1;
+  return 0;
+}
+
+template <typename real>
+int stage(Oracle<real>& o, const std::string& n) {
+  if (n == "zero_details") o.zero_details();
+  else if (n == "maxes") o.maxes_from_assem();
+  else if (n == "reinsert_assem") o.reinsert_assem(o.assem);
+  else if (n == "reinsert_buf") o.reinsert_assem(o.buf);
+  else if (n == "reinsert_point_srcs") o.reinsert_point_srcs();
+  else if (n == "encode") o.encode_flow(false);
+  else if (n == "encode_nghbrs") o.encode_flow(true);
+  else if (n == "regularisation") o.regularisation();
+  else if (n == "extra_significance") o.extra_significance();
+  else if (n == "decoding") o.decoding();
+  else if (n == "traverse") o.traverse();
+  else if (n == "rev_z_order_act_idcs") o.rev_z_order_act_idcs();
+  else if (n == "find_neighbours") o.find_neighbours();
+  else if (n == "compaction_flags") o.compaction_flags();
+  else if (n == "sort_neighbours") o.sort_neighbours_z_order();
+  else if (n == "compaction") o.compaction();
+  else if (n == "rebuild") o.rebuild();
+  else if (n == "load_soln") o.load_soln(o.assem);
+  else if (n == "load_soln_buf") o.load_soln(o.buf);
+  else if (n == "ghost") o.add_ghost_cells(o.assem);
+  else if (n == "ghost_buf") o.add_ghost_cells(o.buf);
+  else if (n == "friction") o.friction();
+  else if (n == "dg2_stage1") o.dg2_update(o.assem, o.buf, false);
+  else if (n == "dg2_stage2") o.dg2_update(o.buf, o.assem, true);
+  else if (n == "fv1") o.fv1_update();
+  else if (n == "limit") o.limit_slopes(o.assem, o.mx_h);
+  else return -1;
+  return 0;
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* mwo_last_error() { return g_err.c_str(); }
+
+void* mwo_create(const char* par, int prec, int argc, const char* const* argv) {
+  try {
+    auto h = std::make_unique<Handle>();
+    h->prec = prec;
+    if (prec == 4) { h->f = std::make_unique<Oracle<float>>(); h->f->load(par, argc, argv); }
+    else { h->d = std::make_unique<Oracle<double>>(); h->d->load(par, argc, argv); }
+    return h.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void mwo_destroy(void* h) { delete static_cast<Handle*>(h); }
+
+int mwo_step(void* hv) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? (int)h->f->step() : (int)h->d->step();
+}
+
+int mwo_steps(void* hv, int n) {
+  int k = 0;
+  for (; k < n; ++k)
+    if (!mwo_step(hv)) break;
+  return k;
+}
+
+int mwo_stage(void* hv, const char* name) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? stage(*h->f, name) : stage(*h->d, name);
+}
+
+long mwo_get(void* hv, const char* name, void* out, long n) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? get_set(*h->f, name, out, n, false) : get_set(*h->d, name, out, n, false);
+}
+
+long mwo_set(void* hv, const char* name, const void* in, long n) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? get_set(*h->f, name, const_cast<void*>(in), n, true)
+                      : get_set(*h->d, name, const_cast<void*>(in), n, true);
+}
+
+double mwo_scalar(void* hv, const char* name) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? scalar(*h->f, name) : scalar(*h->d, name);
+}
+
+int mwo_set_scalar(void* hv, const char* name, double v) {
+  Handle* h = static_cast<Handle*>(hv);
+  return h->prec == 4 ? set_scalar(*h->f, name, v) : set_scalar(*h->d, name, v);
+}
+
+// --- stateless element routines, for the known-answer tests ---------------
+int mwo_morton(int x, int y) { return morton(x, y); }
+int mwo_compact(int code) { return compact(code); }
+int mwo_lvl_idx(int level) { return lvl_idx(level); }
+double mwo_cbrt_d(double x) { return cbrt_newton<double>(x); }
+float mwo_cbrt_f(float x) { return cbrt_newton<float>(x); }
+
+// children[4][3] (child c, mode m) -> out[12] = s0,s1x,s1y, a0,a1x,a1y, b.., g..
+void mwo_encode_mw_d(const double* ch, double* out) {
+  Kids<double> k;
+  for (int c = 0; c < 4; ++c)
+    for (int m = 0; m < 3; ++m) k.u[c][m] = ch[3 * c + m];
+  Coded<double> e = encode_mw(k);
+  for (int m = 0; m < 3; ++m) out[m] = e.s[m];
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 3; ++m) out[3 + 3 * a + m] = e.d[a][m];
+}
+void mwo_encode_mw_f(const float* ch, float* out) {
+  Kids<float> k;
+  for (int c = 0; c < 4; ++c)
+    for (int m = 0; m < 3; ++m) k.u[c][m] = ch[3 * c + m];
+  Coded<float> e = encode_mw(k);
+  for (int m = 0; m < 3; ++m) out[m] = e.s[m];
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 3; ++m) out[3 + 3 * a + m] = e.d[a][m];
+}
+void mwo_decode_mw_d(const double* in, double* ch) {
+  double s[3] = {in[0], in[1], in[2]}, d[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 3; ++m) d[a][m] = in[3 + 3 * a + m];
+  Kids<double> k = decode_mw(s, d);
+  for (int c = 0; c < 4; ++c)
+    for (int m = 0; m < 3; ++m) ch[3 * c + m] = k.u[c][m];
+}
+void mwo_decode_mw_f(const float* in, float* ch) {
+  float s[3] = {in[0], in[1], in[2]}, d[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < 3; ++m) d[a][m] = in[3 + 3 * a + m];
+  Kids<float> k = decode_mw(s, d);
+  for (int c = 0; c < 4; ++c)
+    for (int m = 0; m < 3; ++m) ch[3 * c + m] = k.u[c][m];
+}
+void mwo_encode_hw_d(const double* c, double* out) { encode_hw(c, out[0], out[1], out[2], out[3]); }
+void mwo_decode_hw_d(const double* in, double* c) { decode_hw(in[0], in[1], in[2], in[3], c); }
+// filter literal table as doubles: [family HH,GA,GB,GC][k][i][j]
+void mwo_filters_d(double* out) {
+  const Lit(*fam[4])[3][3] = {FHH, FGA, FGB, FGC};
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[((f * 4 + k) * 3 + i) * 3 + j] = fam[f][k][i][j].d;
+}
+void mwo_filters_f(float* out) {
+  const Lit(*fam[4])[3][3] = {FHH, FGA, FGB, FGC};
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[((f * 4 + k) * 3 + i) * 3 + j] = fam[f][k][i][j].f;
+}
+// HLL x-flux of (h,qx,qy) pairs -> out[3]
+void mwo_flux_hll_x_d(const double* Un, const double* Up, double tol_h, double g, double* out) {
+  FV<double> F = flux_hll_x<double>({Un[0], Un[1], Un[2]}, {Up[0], Up[1], Up[2]}, tol_h, g);
+  out[0] = F.h; out[1] = F.qx; out[2] = F.qy;
+}
+void mwo_flux_hll_y_d(const double* Un, const double* Up, double tol_h, double g, double* out) {
+  FV<double> F = flux_hll_y<double>({Un[0], Un[1], Un[2]}, {Up[0], Up[1], Up[2]}, tol_h, g);
+  out[0] = F.h; out[1] = F.qx; out[2] = F.qy;
+}
+void mwo_friction_d(double h, double tol_h, double tol_s, double* q, double n, double g, double dt) {
+  apply_friction<double>(h, tol_h, tol_s, q[0], q[1], n, g, dt);
+}
+int mwo_omp_threads() {
+#ifdef _OPENMP
+  extern int omp_get_max_threads(void);
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+// ============================================================================
+// CPU ORACLE for the adaptive MWDG2 / HWFV1 shallow-water time step.
+//
+// TEST INFRASTRUCTURE ONLY. This file restates, on the host, the algorithm of
+// the reference step loop (source/run_simulation.cu:226-774) and its kernels,
+// element by element, in the reference's own data layout (N-sized Z-ordered
+// finest arrays, full detail arrays, CUB-style compaction). It is the checker
+// for the HIP product path; nothing in the product links or calls it. Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+//
+// Every function names the reference file:line it follows. Arithmetic is done
+// in the order written by the reference, compiled with -ffp-contract=off, so
+// that an implementation doing the same operations in the same order agrees
+// bit for bit. Two deliberate, documented deviations:
+//   * h^(1/3) (friction, FREE ghost) is evaluated with cbrt_newton() below
+//     (a fixed Newton iteration from a bit-level seed) instead of pow(h,1/3),
+//     so host and device produce identical bits; the two agree to ~1 ulp.
+//   * time-series lookup holds the last value once the row index would step
+//     past the table end (the reference reads past the array there,
+//     classes/Boundary.h:476-496).
+// For L < 6 the reference's regularisation reads the mask array out of
+// bounds (mra/regularisation.cu:10-26 launches a single block from level 4);
+// here regularisation always covers parent levels L-2..0, which equals the
+// reference for every L >= 6.
+//
+// Pinning (see tests/test_oracle_*.py): filters against their closed forms,
+// orthogonality and perfect reconstruction; Morton/sort against closed forms
+// and brute force; masks and leaves against a brute-force tree recursion and
+// invariants; neighbours against brute force; HLL against Toro's textbook
+// cases; lake-at-rest well-balancedness; closed-wall mass conservation.
+// ============================================================================
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mwo {
+
+enum SolverType { HWFV1 = 1, MWDG2 = 2 };                         // types/SolverTypes.h:3-7
+enum InletType  { CLOSED = 1, FREE = 2, HFIX = 4, HVAR = 8, QFIX = 16, QVAR = 32 }; // types/InletTypes.h:3-11
+enum Direction  { NORTH = 1, EAST = 2, SOUTH = 4, WEST = 8 };     // types/Directions.h:3-9
+
+// utilities/get_lvl_idx.cuh:11-24 (3-entry pad)
+inline int lvl_idx(int level) { return ((1 << (2 * level)) - 1) / 3 + 3; }
+
+// zorder/dilate.cuh:25-41
+inline int dilate(int c) {
+  c &= 0x0000ffff;
+  c = (c ^ (c << 8)) & 0x00ff00ff;
+  c = (c ^ (c << 4)) & 0x0f0f0f0f;
+  c = (c ^ (c << 2)) & 0x33333333;
+  c = (c ^ (c << 1)) & 0x55555555;
+  return c;
+}
+// zorder/compact.cuh:9-21
+inline int compact(int code) {
+  int c = code & 0x55555555;
+  c = (c ^ (c >> 1)) & 0x33333333;
+  c = (c ^ (c >> 2)) & 0x0f0f0f0f;
+  c = (c ^ (c >> 4)) & 0x00ff00ff;
+  c = (c ^ (c >> 8)) & 0x0000ffff;
+  return c;
+}
+// zorder/generate_morton_code.cuh:7-16
+inline int morton(int x, int y) { return dilate(x) | (dilate(y) << 1); }
+
+// Cube root used in place of pow(x, 1/3) (apply_friction.cuh:41,
+// add_ghost_cells.cu:135). Seed from the exponent bits, then Newton steps
+// y <- (2y + x/y^2)/3. Same operations, same order on host and device.
+template <typename real> inline real cbrt_newton(real x);
+template <> inline double cbrt_newton<double>(double x) {
+  if (!(x > 0.0)) return (x == 0.0) ? 0.0 : std::nan("");
+  if (std::isinf(x)) return x;
+  uint64_t b; std::memcpy(&b, &x, 8);
+  const bool sub = (b >> 52) == 0;
+  double xs = sub ? x * 18014398509481984.0 : x;  // 2^54 for subnormals
+  std::memcpy(&b, &xs, 8);
+  b = b / 3 + 0x2A9F7893782DA1CEull;
+  double y; std::memcpy(&y, &b, 8);
+  for (int it = 0; it < 6; ++it) y = (2.0 * y + xs / (y * y)) / 3.0;
+  return sub ? y / 262144.0 : y;                    // 2^18 = cbrt(2^54)
+}
+template <> inline float cbrt_newton<float>(float x) {
+  if (!(x > 0.0f)) return (x == 0.0f) ? 0.0f : std::nanf("");
+  if (std::isinf(x)) return x;
+  uint32_t b; std::memcpy(&b, &x, 4);
+  const bool sub = (b >> 23) == 0;
+  float xs = sub ? x * 16777216.0f : x;             // 2^24 for subnormals
+  std::memcpy(&b, &xs, 4);
+  b = b / 3 + 0x2A51067Fu;
+  float y; std::memcpy(&y, &b, 4);
+  for (int it = 0; it < 5; ++it) y = (2.0f * y + xs / (y * y)) / 3.0f;
+  return sub ? y / 256.0f : y;                      // 2^8 = cbrt(2^24)
+}
+
+// ---------------------------------------------------------------------------
+// Input parsing: a restatement of the keyword readers (prefix match of the
+// first token, input/read_keyword_real.cpp:30-40, read_keyword_int.cpp,
+// read_keyword_bool.cpp:28-36, read_keyword_str.cpp).
+// ---------------------------------------------------------------------------
+bool   kw_found(const char* file, const char* kw);
+double kw_real(const char* file, const char* kw, bool as_float);
+int    kw_int(const char* file, const char* kw);
+std::string kw_str(const char* file, const char* kw);
+
+struct TimeSeries {                       // classes/Boundary.h:376-469
+  std::vector<double> time, value;        // stored at double, cast on use
+};
+
+template <typename real>
+struct Params {
+  // SolverParams (classes/SolverParams.h:20-34, SolverParams.cu:10-47)
+  int  L = 0;
+  real initial_tstep = 0, epsilon = 0, wall_height = 0;
+  real tol_h = real(1e-3), tol_q = real(0.0), tol_s = real(1e-9);
+  int  solver = 0;
+  real CFL = 0;
+  bool limitslopes = false;
+  real tol_Krivo = real(9999.0);
+  bool refine_wall = false;
+  int  ref_thickness = 0;
+  bool startq2d = false;
+  // SimulationParams for test_case 0 (classes/SimulationParams.cpp:17-33)
+  real xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+  int  xsz = 0, ysz = 0;
+  real g = real(9.80665), time = 0, manning = 0;
+  bool is_monai = false, is_oregon = false;
+  real dx = 0;                            // input/read_cell_size.cpp:3-45
+};
+
+template <typename real>
+struct Bdy {                               // classes/Boundary.h:17-501
+  int start = 0, end = 0, bdytype = CLOSED;
+  real inlet = 0;
+  int row = 0;
+  std::vector<real> tdata, vdata;
+  std::vector<int> codes;                  // for preflagging only
+  bool bound(int c) const { return c >= start && c <= end; }
+  real q_src(real dt, real dx) const { return inlet * dt / dx; }
+};
+
+template <typename real>
+struct PointSrc {                          // classes/PointSources.h:15-473
+  int code = 0, type = 0, row = 0;
+  real value = 0;
+  std::vector<real> tdata, vdata;
+};
+
+// Linear interpolation with the reference's one-row-per-call advance
+// (Boundary.h:471-500, PointSources.h:420-452).
+template <typename real>
+inline void advance_series(real current_time, int& row, const std::vector<real>& t,
+                           const std::vector<real>& v, real& out) {
+  const int n = (int)t.size();
+  if ((row - 1) < n && row + 1 < n) {
+    real t_1 = t[row];
+    real t_2 = t[row + 1];
+    if (current_time > t_2 && row + 2 < n) {
+      row++;
+      t_1 = t[row];
+      t_2 = t[row + 1];
+    } else if (current_time > t_2) {
+      out = v[n - 1];
+      return;
+    }
+    real i_1 = v[row];
+    real i_2 = v[row + 1];
+    out = i_1 + (i_2 - i_1) / (t_2 - t_1) * (current_time - t_1);
+  } else {
+    out = v[n - 1];
+  }
+}
+
+// Per-element arrays of one solution set (classes/AssembledSolution.h:69-91).
+template <typename real>
+struct Sol {
+  int length = 0;
+  std::vector<real> h0, h1x, h1y, qx0, qx1x, qx1y, qy0, qy1x, qy1y, z0, z1x, z1y;
+  std::vector<int> act, lev;
+  void alloc(int n) {
+    length = n;
+    for (auto* v : {&h0, &h1x, &h1y, &qx0, &qx1x, &qx1y, &qy0, &qy1x, &qy1y, &z0, &z1x, &z1y})
+      v->assign(n, real(0));
+    act.assign(n, 0);
+    lev.assign(n, 0);
+  }
+};
+
+struct Maxes { double eta, h, qx, qy, z; };
+
+template <typename real>
+struct Oracle {
+  Params<real> p;
+  int test_case = 0;
+  Bdy<real> bN, bE, bS, bW;
+  std::vector<PointSrc<real>> srcs;
+  std::vector<int> stage_codes;
+
+  int N = 0, D = 0, H = 0, mesh_dim = 0;
+  // hierarchy (classes/ScaleCoefficients.h:47-60)
+  std::vector<real> eta0, eta1x, eta1y, qx0, qx1x, qx1y, qy0, qy1x, qy1y, z0, z1x, z1y;
+  // details: [var 0..3 = eta,qx,qy,z][mode 0..2 = 0,1x,1y][alpha,beta,gamma]
+  std::vector<real> det[4][3][3];
+  std::vector<real> norm;
+  std::vector<uint8_t> sig, preflag;
+  Sol<real> assem, buf;
+  Sol<real> nb[4], bufnb[4];                  // index 0..3 = N,E,S,W
+  std::vector<int> morton_codes, rev_z_order, rev_row_major;
+  std::vector<uint8_t> flag_ne, flag_sw;
+  std::vector<real> dt_cfl;
+
+  real dt = real(0.001);                      // run_simulation.cu:37
+  real current_time = 0;
+  bool first = true;
+  int timestep = 1;
+  real mx_eta = 1, mx_h = 1, mx_qx = 1, mx_qy = 1, mx_z = 1;
+  int last_P_e = 0, last_P_d = 0;             // parents encoded / decoded (diagnostics)
+
+  // --- set-up -------------------------------------------------------------
+  void load(const char* par, int argc, const char* const* argv);
+  void init_state();                           // run_simulation.cu:112-223
+  // --- stages (each mirrors one reference kernel / wrapper) ---------------
+  void maxes_from_assem();                     // mra/get_max_scale_coeffs.cu:3-27
+  void reinsert_assem(const Sol<real>& s);     // mra/reinsert_assem_sol.cu:3-32
+  void update_srcs(real t);                    // classes/PointSources.h:454-472
+  void reinsert_point_srcs();                  // mra/reinsert_point_srcs.cu:3-29
+  void zero_details();                         // mra/zero_details.cu:3-53
+  void encode_flow(bool for_nghbrs);           // mra/encode_flow.cu:3-92
+  void regularisation();                       // mra/regularisation.cu:3-27
+  void extra_significance();                   // mra/extra_significance.cu:3-33
+  void decoding();                             // mra/decoding.cu:3-67
+  void traverse();                             // mra/traverse_tree_of_sig_details.cu:3-101
+  void rev_z_order_act_idcs();                 // zorder/rev_z_order_act_idcs.cu:3-20
+  void find_neighbours();                      // neighbours/find_neighbours.cu:3-430
+  void compaction_flags();                     // neighbours/get_compaction_flags.cu:3-38
+  void sort_neighbours_z_order();              // zorder/sort_neighbours_z_order.cu:3-31
+  void compaction();                           // neighbours/compaction.cu:3-203
+  void load_soln(Sol<real>& s);                // neighbours/load_soln_and_nghbr_coeffs.cu:3-104
+  void update_inlets(real t);                  // classes/Boundaries.h:28-38
+  void add_ghost_cells(Sol<real>& s);          // neighbours/add_ghost_cells.cu:3-499
+  void friction();                             // operators/friction_implicit.cu:3-265
+  void limit_slopes(Sol<real>& s, real max_h); // operators/limit_slopes.cu:3-238
+  void dg2_update(Sol<real>& load, Sol<real>& store, bool rkdg2); // operators/dg2_update.cu:3-359
+  void fv1_update();                           // operators/fv1_update.cu:3-243
+  real dt_min();                               // operators/get_dt_CFL.cu:3-50
+  void rebuild();                              // run_simulation.cu:274-466
+  bool step();                                 // one pass of run_simulation.cu:226-774
+  bool done() const { return !(current_time < p.time); }
+};
+
+}  // namespace mwo

@@ -1,0 +1,201 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+The oracle restates the reference step loop (source/run_simulation.cu:226-774)
+on the host; see mw_oracle.hpp for what each C++ routine follows. Only the
+tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle library with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, cp, i, d, l = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        L.mwo_create.restype = vp
+        L.mwo_create.argtypes = [cp, i, i, ctypes.POINTER(cp)]
+        L.mwo_destroy.argtypes = [vp]
+        L.mwo_step.argtypes = [vp]
+        L.mwo_steps.argtypes = [vp, i]
+        L.mwo_stage.argtypes = [vp, cp]
+        L.mwo_get.restype = l
+        L.mwo_get.argtypes = [vp, cp, vp, l]
+        L.mwo_set.restype = l
+        L.mwo_set.argtypes = [vp, cp, vp, l]
+        L.mwo_scalar.restype = d
+        L.mwo_scalar.argtypes = [vp, cp]
+        L.mwo_set_scalar.argtypes = [vp, cp, d]
+        L.mwo_last_error.restype = cp
+        L.mwo_cbrt_d.restype = d
+        L.mwo_cbrt_d.argtypes = [d]
+        L.mwo_cbrt_f.restype = ctypes.c_float
+        L.mwo_cbrt_f.argtypes = [ctypes.c_float]
+        for n in ("mwo_encode_mw_d", "mwo_decode_mw_d", "mwo_encode_hw_d", "mwo_decode_hw_d",
+                  "mwo_encode_mw_f", "mwo_decode_mw_f"):
+            getattr(L, n).argtypes = [vp, vp]
+        L.mwo_filters_d.argtypes = [vp]
+        L.mwo_filters_f.argtypes = [vp]
+        L.mwo_flux_hll_x_d.argtypes = [vp, vp, d, d, vp]
+        L.mwo_flux_hll_y_d.argtypes = [vp, vp, d, d, vp]
+        L.mwo_friction_d.argtypes = [d, d, d, vp, d, d, d]
+        _lib = L
+    return _lib
+
+
+_INT_FIELDS = ("act", "lev")
+_BYTE_ARRAYS = ("sig", "preflag", "flag_ne", "flag_sw")
+_INT_ARRAYS = ("rev_z_order", "rev_row_major", "morton")
+
+
+class Oracle:
+    """One oracle simulation built from a .par file (test_case 0 inputs)."""
+
+    def __init__(self, par: str, precision: str = "f64", argv=()):
+        self.prec = 4 if precision in ("f32", "float32", 4) else 8
+        self.real = np.float32 if self.prec == 4 else np.float64
+        args = (ctypes.c_char_p * max(1, len(argv)))(*[a.encode() for a in argv])
+        self._h = lib().mwo_create(par.encode(), self.prec, len(argv), args)
+        if not self._h:
+            raise RuntimeError("oracle: " + lib().mwo_last_error().decode())
+
+    def close(self):
+        if self._h:
+            lib().mwo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _dtype(self, name: str):
+        if name in _BYTE_ARRAYS:
+            return np.uint8
+        if name in _INT_ARRAYS or name.split(".")[-1] in _INT_FIELDS:
+            return np.int32
+        return self.real
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().mwo_get(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        a = np.empty(n, dtype=self._dtype(name))
+        lib().mwo_get(self._h, name.encode(), a.ctypes.data, n)
+        return a
+
+    def set(self, name: str, values) -> None:
+        a = np.ascontiguousarray(values, dtype=self._dtype(name))
+        if lib().mwo_set(self._h, name.encode(), a.ctypes.data, a.size) < 0:
+            raise KeyError(name)
+
+    def scalar(self, name: str) -> float:
+        return lib().mwo_scalar(self._h, name.encode())
+
+    def set_scalar(self, name: str, v: float) -> None:
+        if lib().mwo_set_scalar(self._h, name.encode(), float(v)) != 0:
+            raise KeyError(name)
+
+    def stage(self, name: str) -> None:
+        if lib().mwo_stage(self._h, name.encode()) != 0:
+            raise KeyError(name)
+
+    def step(self) -> bool:
+        return bool(lib().mwo_step(self._h))
+
+    def steps(self, n: int) -> int:
+        return lib().mwo_steps(self._h, n)
+
+    # convenience views -------------------------------------------------
+    def leaves(self):
+        n = int(self.scalar("length"))
+        return self.get("assem.act")[:n].copy()
+
+    def neighbours(self):
+        n = int(self.scalar("length"))
+        return np.stack([self.get(f"nb{k}.act")[:n] for k in range(4)])
+
+
+# stateless element routines -----------------------------------------------
+def encode_mw(children: np.ndarray, dtype=np.float64) -> np.ndarray:
+    c = np.ascontiguousarray(children, dtype=dtype).reshape(12)
+    out = np.zeros(12, dtype=dtype)
+    fn = lib().mwo_encode_mw_d if dtype == np.float64 else lib().mwo_encode_mw_f
+    fn(c.ctypes.data, out.ctypes.data)
+    return out
+
+
+def decode_mw(coded: np.ndarray, dtype=np.float64) -> np.ndarray:
+    c = np.ascontiguousarray(coded, dtype=dtype).reshape(12)
+    out = np.zeros(12, dtype=dtype)
+    fn = lib().mwo_decode_mw_d if dtype == np.float64 else lib().mwo_decode_mw_f
+    fn(c.ctypes.data, out.ctypes.data)
+    return out
+
+
+def encode_hw(children) -> np.ndarray:
+    c = np.ascontiguousarray(children, dtype=np.float64)
+    out = np.zeros(4)
+    lib().mwo_encode_hw_d(c.ctypes.data, out.ctypes.data)
+    return out
+
+
+def decode_hw(coded) -> np.ndarray:
+    c = np.ascontiguousarray(coded, dtype=np.float64)
+    out = np.zeros(4)
+    lib().mwo_decode_hw_d(c.ctypes.data, out.ctypes.data)
+    return out
+
+
+def filters(dtype=np.float64) -> np.ndarray:
+    """[family HH,GA,GB,GC][filter k][row i][col j] as in mra/Filters.h."""
+    out = np.zeros(4 * 4 * 3 * 3, dtype=dtype)
+    (lib().mwo_filters_d if dtype == np.float64 else lib().mwo_filters_f)(out.ctypes.data)
+    return out.reshape(4, 4, 3, 3)
+
+
+def flux_hll(Un, Up, tol_h=1e-3, g=9.80665, axis="x") -> np.ndarray:
+    a = np.ascontiguousarray(Un, dtype=np.float64)
+    b = np.ascontiguousarray(Up, dtype=np.float64)
+    out = np.zeros(3)
+    fn = lib().mwo_flux_hll_x_d if axis == "x" else lib().mwo_flux_hll_y_d
+    fn(a.ctypes.data, b.ctypes.data, tol_h, g, out.ctypes.data)
+    return out
+
+
+def friction(h, qx, qy, n, dt, tol_h=1e-3, tol_s=1e-9, g=9.80665):
+    q = np.array([qx, qy], dtype=np.float64)
+    lib().mwo_friction_d(h, tol_h, tol_s, q.ctypes.data, n, g, dt)
+    return q
+
+
+def cbrt(x, dtype=np.float64):
+    return lib().mwo_cbrt_d(float(x)) if dtype == np.float64 else lib().mwo_cbrt_f(float(x))
+
+
+def morton(x: int, y: int) -> int:
+    return lib().mwo_morton(x, y)
+
+
+def compact(code: int) -> int:
+    return lib().mwo_compact(code)
+
+
+def lvl_idx(level: int) -> int:
+    return lib().mwo_lvl_idx(level)
