@@ -203,7 +203,7 @@ def random_case(d: str, L: int, seed: int, solver: str = "mwdg2", eps: float = 1
     if srcs:
         us = uniform(seed + 7, 2 * srcs)
         for s in range(srcs):
-            lines.append(f"P {int(us[2 * s] * xs) + 0.5!r} {int(us[2 * s + 1] * ys) + 0.5!r} QFIX 0.5")
+            lines.append(f"P {float(int(us[2 * s] * xs) + 0.5)!r} {float(int(us[2 * s + 1] * ys) + 0.5)!r} QFIX 0.5")
     if lines:
         with open(os.path.join(d, "case.bci"), "w") as f:
             f.write("\n".join(lines) + "\n")
@@ -217,7 +217,7 @@ def random_case(d: str, L: int, seed: int, solver: str = "mwdg2", eps: float = 1
         with open(os.path.join(d, "case.stage"), "w") as f:
             f.write(f"{stage}\n")
             for s in range(stage):
-                f.write(f"{us[2 * s] * xs!r} {us[2 * s + 1] * ys!r}\n")
+                f.write(f"{float(us[2 * s] * xs)!r} {float(us[2 * s + 1] * ys)!r}\n")
         extra.append(("stagefile", "case.stage"))
     if refine_wall:
         extra.extend(["refine_wall", ("ref_thickness", refine_wall)])

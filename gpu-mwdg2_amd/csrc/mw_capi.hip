@@ -63,7 +63,7 @@ struct CtxBase {
   virtual ~CtxBase() = default;
   int prec = 8;
   hipStream_t stream = nullptr;
-  bool profiling = false;
+  int profiling = 0;                // 0 off, 1 all kernels, 2 update kernels only
   struct Prof { long long n = 0; double ms = 0; };
   std::map<std::string, Prof> prof;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -77,7 +77,8 @@ struct CtxBase {
   }
   template <typename F>
   void timed(const char* name, F&& launch) {
-    if (!profiling) { launch(); return; }
+    const bool update = !std::strncmp(name, "stage", 5) || !std::strncmp(name, "fv1", 3);
+    if (!profiling || (profiling == 2 && !update)) { launch(); return; }
     hipEvent_t a = ev(), b = ev();
     MW_HIP(hipEventRecord(a, stream));
     launch();
@@ -556,6 +557,10 @@ struct Ctx : CtxBase {
     c->solver = hc.solver;
     c->precision = (int)sizeof(real);
     for (int k = 0; k < 5; ++k) c->maxes[k] = s.mx[k];
+    c->leaf_updates = (long long)s.leaf_sum;
+    c->parents_encoded = (long long)s.pe_sum;
+    c->parents_decoded = (long long)s.pd_sum;
+    c->parents_rescaled = (long long)s.ps_sum;
   }
 
   void stage(const std::string& s) override {
@@ -875,7 +880,7 @@ int mw_profile(mw_ctx* ctx, int enable) {
   MW_NEED(ctx);
   return guarded([&] {
     ctx->impl->drain_profile();
-    ctx->impl->profiling = enable != 0;
+    ctx->impl->profiling = enable;
     if (enable) ctx->impl->prof.clear();
   });
 }

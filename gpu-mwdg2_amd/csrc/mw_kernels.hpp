@@ -25,6 +25,11 @@ struct DevState {
   real inlet[4];                  // N, E, S, W boundary inlet values
   int row[4];                     // time-series cursors
   int n_src;
+  // work counters summed over steps (for the algorithmic-byte model)
+  unsigned long long leaf_sum;    // leaves updated
+  unsigned long long pe_sum;      // parents encoded (previous tree)
+  unsigned long long pd_sum;      // parents decoded (new tree)
+  unsigned long long ps_sum;      // parents re-encoded between RK stages
 };
 
 // Time series of the boundaries (entries 0..3) and point sources (4..).

@@ -32,7 +32,9 @@ class Counters(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("sim_time", ctypes.c_double), ("steps", ctypes.c_longlong), ("n_leaves", ctypes.c_longlong),
                 ("done", ctypes.c_int), ("max_ref_lvl", ctypes.c_int), ("solver", ctypes.c_int),
-                ("precision", ctypes.c_int), ("maxes", ctypes.c_double * 5)]
+                ("precision", ctypes.c_int), ("maxes", ctypes.c_double * 5),
+                ("leaf_updates", ctypes.c_longlong), ("parents_encoded", ctypes.c_longlong),
+                ("parents_decoded", ctypes.c_longlong), ("parents_rescaled", ctypes.c_longlong)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "maxes"}
@@ -165,8 +167,9 @@ class Simulation:
         _check(lib().mw_set_initial(self._h, ptrs))
 
     # profiling -----------------------------------------------------------
-    def profile(self, enable: bool = True):
-        _check(lib().mw_profile(self._h, int(enable)))
+    def profile(self, level: int = 1):
+        """0 off, 1 every kernel, 2 only the cell-update kernels."""
+        _check(lib().mw_profile(self._h, int(level)))
 
     def profile_read(self) -> dict:
         n = ctypes.c_int()

@@ -45,6 +45,11 @@ typedef struct mw_counters {
   int solver;                    /* 1 = HWFV1, 2 = MWDG2                           */
   int precision;
   double maxes[5];               /* eta, h, qx, qy, z used by the last encode      */
+  /* work summed over all steps taken (inputs of the algorithmic-byte model)  */
+  long long leaf_updates;        /* leaves updated (per step: N_a)                  */
+  long long parents_encoded;     /* parents of the previous tree encoded (P_e)      */
+  long long parents_decoded;     /* significant parents decoded (P_d)               */
+  long long parents_rescaled;    /* parents re-encoded between the RK stages        */
 } mw_counters;
 
 typedef struct mw_boundary {     /* classes/Boundary.h:17-31, one side            */
@@ -115,6 +120,9 @@ int mw_update_stage(mw_ctx* ctx, int stage); /* 1: load_soln+ghosts+friction+(li
                                                 0: HWFV1 load_soln+ghosts+friction+fv1_update+get_dt_CFL */
 int mw_encode_scale(mw_ctx* ctx);        /* reinsert + encode_flow(for_nghbrs=true)               */
 int mw_cfl_min(mw_ctx* ctx, double* dt_next); /* operators/get_dt_CFL.cu:3-50 (reads the device min) */
+/* Any of the above by name: "prologue", "point_sources", "encode", "decode",
+ * "emit", "swap_masks", "stage1", "fv1", "encode_scale", "stage2". */
+int mw_stage_named(mw_ctx* ctx, const char* name);
 
 /* Named device arrays: hierarchy "A.eta0".."A.qy1y", "B.*", "z0","z1x","z1y";
  * masks "sig" (current tree), "sig_next", "xflag", "leaflev", "preflag";
@@ -129,7 +137,8 @@ int mw_set_scalar(mw_ctx* ctx, const char* name, double v);
  * done on the GPU (cub::DeviceRadixSort::SortPairs, get_sorting_indices.cu:24-57). */
 int mw_sort_pairs(int n, const int* keys, const int* vals, int* keys_out, int* vals_out);
 
-/* Per-kernel timing with HIP events on the context stream. */
+/* Per-kernel timing with HIP events on the context stream: enable = 0 off,
+ * 1 every kernel, 2 only the cell-update kernels (stage1, stage2, fv1). */
 int mw_profile(mw_ctx* ctx, int enable);
 int mw_profile_read(mw_ctx* ctx, mw_kernel_time* out, int max_entries, int* n_entries);
 

@@ -39,8 +39,8 @@ def compare_step(o: oracle.Oracle, s, mw: bool, tol: float = 0.0, label: str = "
         bad = np.flatnonzero(nb_o[k] != g)
         assert bad.size == 0, f"{label}: neighbour {k} differs at {bad[:10]}: {nb_o[k][bad[:5]]} vs {g[bad[:5]]}"
     D = int(o.scalar("D"))
-    sig_o = o.get("sig")[:D]
-    sig_g = s.get("sig")[:D]
+    sig_o = o.get("sig")[3:D]          # entries 0..2 are the index pad
+    sig_g = s.get("sig")[3:D]
     assert np.array_equal(sig_o, sig_g), f"{label}: masks differ at {np.flatnonzero(sig_o != sig_g)[:10]}"
     st = oracle_leaf_state(o, mw)
     worst = 0.0
@@ -79,7 +79,7 @@ def run_pair(par: str, precision: str, steps: int, tol: float = 0.0, check_every
             assert np.array_equal(a, b), f"initial {name} differs"
         for name in ("z0",) + (("z1x", "z1y") if mw else ()):
             assert np.array_equal(o.get(name), s.get(name)), f"z hierarchy {name} differs"
-        assert np.array_equal(o.get("preflag")[: int(o.scalar("D"))], s.get("preflag")[: int(o.scalar("D"))]), "preflags differ"
+        assert np.array_equal(o.get("preflag")[3: int(o.scalar("D"))], s.get("preflag")[3: int(o.scalar("D"))]), "preflags differ"
         reps = []
         for k in range(steps):
             more = o.step()
