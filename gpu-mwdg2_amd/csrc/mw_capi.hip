@@ -129,6 +129,8 @@ struct Ctx : CtxBase {
   int *rev_z = nullptr, *rev_rm = nullptr;
   real* uprev = nullptr;
   unsigned long long* maxh2 = nullptr;
+  real* red = nullptr;                        // [5][RED_SLOTS] next-step reduction slots
+  unsigned long long* cnt[3] = {nullptr, nullptr, nullptr};  // P_e, P_d, rescaled parents (slot arrays)
   // series and sources on the device
   int *s_off = nullptr, *s_len = nullptr;
   real *s_t = nullptr, *s_v = nullptr;
@@ -192,6 +194,11 @@ struct Ctx : CtxBase {
     rev_rm = alloc<int>(N);
     st = alloc<DevState<real>>(1);
     maxh2 = alloc<unsigned long long>(1);
+    red = alloc<real>((size_t)5 * RED_SLOTS);
+    for (int q = 0; q < 3; ++q) {
+      cnt[q] = alloc<unsigned long long>(CNT_SLOTS);
+      MW_HIP(hipMemsetAsync(cnt[q], 0, sizeof(unsigned long long) * CNT_SLOTS, stream));
+    }
     if (hc.limitslopes && mwm) uprev = alloc<real>((size_t)9 * N);
     fill_params();
     upload_sources_and_series();
@@ -403,7 +410,6 @@ struct Ctx : CtxBase {
     s.dt = real(0.001);                                      // run_simulation.cu:37
     s.sim_time = hc.sim_time;
     for (int k = 0; k < 5; ++k) s.mx[k] = mxs[k];
-    s.dt_acc = 0;
     s.n_leaves = N;
     s.done = 0;
     s.first = 1;
@@ -418,7 +424,9 @@ struct Ctx : CtxBase {
 
   // ---- step pieces ---------------------------------------------------------
   void k_prologue_launch() {
-    timed("prologue", [&] { hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(1), 0, stream, st, P, series(), sources()); });
+    timed("prologue", [&] {
+      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(WG), 0, stream, st, P, series(), sources(), red, leaf_grid());
+    });
   }
   void k_sources_launch() {
     if (!n_src) return;
@@ -434,10 +442,10 @@ struct Ctx : CtxBase {
       timed("encode", [&] {
         if (mwm)
           hipLaunchKernelGGL((k_encode<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             preflag, level, L, hc.epsilon, P.eps_extra_factor);
+                             preflag, level, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
         else
           hipLaunchKernelGGL((k_encode<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             preflag, level, L, hc.epsilon, P.eps_extra_factor);
+                             preflag, level, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       });
     }
   }
@@ -449,10 +457,10 @@ struct Ctx : CtxBase {
       timed("decode", [&] {
         if (mwm)
           hipLaunchKernelGGL((k_topdown<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             leaflev, level, L, hc.epsilon, P.eps_extra_factor);
+                             leaflev, level, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
         else
           hipLaunchKernelGGL((k_topdown<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             leaflev, level, L, hc.epsilon, P.eps_extra_factor);
+                             leaflev, level, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
       });
     }
   }
@@ -468,20 +476,20 @@ struct Ctx : CtxBase {
                          nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
     });
   }
-  int leaf_grid() const { return std::min(nblk(N), 256 * 16); }
+  int leaf_grid() const { return std::min(nblk(N), RED_SLOTS); }
   void stage1_launch() {
     const int other = main_h ^ 1;
     timed(mwm ? "stage1" : "fv1", [&] {
       if (mwm) {
         if (hc.limitslopes)
           hipLaunchKernelGGL((k_stage1<real, true, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
-                             hier[main_h], hier[other], zh, uprev);
+                             hier[main_h], hier[other], zh, uprev, red);
         else
           hipLaunchKernelGGL((k_stage1<real, true, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
-                             hier[main_h], hier[other], zh, uprev);
+                             hier[main_h], hier[other], zh, uprev, red);
       } else {
         hipLaunchKernelGGL((k_stage1<real, false, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
-                           hier[main_h], hier[other], zh, uprev);
+                           hier[main_h], hier[other], zh, uprev, red);
       }
     });
     if (!mwm) main_h = other;
@@ -490,7 +498,7 @@ struct Ctx : CtxBase {
     for (int level = L - 1; level >= 0; --level) {
       const long long n = 1LL << (2 * level);
       timed("encode_scale", [&] {
-        hipLaunchKernelGGL((k_encode_scale<real>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[1], sig[cur], level);
+        hipLaunchKernelGGL((k_encode_scale<real>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[1], sig[cur], level, cnt[2]);
       });
     }
   }
@@ -502,12 +510,12 @@ struct Ctx : CtxBase {
       });
       timed("stage2", [&] {
         hipLaunchKernelGGL((k_stage2<real, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
-                           zh, uprev, maxh2);
+                           zh, uprev, maxh2, red);
       });
     } else {
       timed("stage2", [&] {
         hipLaunchKernelGGL((k_stage2<real, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0],
-                           hier[1], zh, uprev, maxh2);
+                           hier[1], zh, uprev, maxh2, red);
       });
     }
   }
@@ -544,11 +552,29 @@ struct Ctx : CtxBase {
     return s;
   }
 
+  // minimum over the reduction slots written by the last update kernel
+  real next_dt() {
+    std::vector<real> v(leaf_grid());
+    MW_HIP(hipMemcpyAsync(v.data(), red, sizeof(real) * v.size(), hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    real m = v[0];
+    for (real x : v) m = x < m ? x : m;
+    return m;
+  }
+  unsigned long long slot_sum(const unsigned long long* d) {
+    std::vector<unsigned long long> v(CNT_SLOTS);
+    MW_HIP(hipMemcpyAsync(v.data(), d, sizeof(unsigned long long) * CNT_SLOTS, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    unsigned long long s = 0;
+    for (auto x : v) s += x;
+    return s;
+  }
+
   void counters(mw_counters* c) override {
     const DevState<real> s = read_state();
     c->time = s.time;
     c->dt = s.dt;
-    c->dt_next = s.first ? (double)s.dt : (double)Bits<real>::from(s.dt_acc);
+    c->dt_next = s.first ? (double)s.dt : (double)next_dt();
     c->sim_time = s.sim_time;
     c->steps = s.steps;
     c->n_leaves = s.n_leaves;
@@ -558,9 +584,9 @@ struct Ctx : CtxBase {
     c->precision = (int)sizeof(real);
     for (int k = 0; k < 5; ++k) c->maxes[k] = s.mx[k];
     c->leaf_updates = (long long)s.leaf_sum;
-    c->parents_encoded = (long long)s.pe_sum;
-    c->parents_decoded = (long long)s.pd_sum;
-    c->parents_rescaled = (long long)s.ps_sum;
+    c->parents_encoded = (long long)slot_sum(cnt[0]);
+    c->parents_decoded = (long long)slot_sum(cnt[1]);
+    c->parents_rescaled = (long long)slot_sum(cnt[2]);
   }
 
   void stage(const std::string& s) override {
@@ -617,7 +643,11 @@ struct Ctx : CtxBase {
     DevState<real> s = read_state();
     if (name == "time") s.time = (real)v;
     else if (name == "dt") s.dt = (real)v;
-    else if (name == "dt_next") s.dt_acc = Bits<real>::to_host((real)v);
+    else if (name == "dt_next") {
+      std::vector<real> slots(leaf_grid(), real(__builtin_inff()));
+      slots[0] = (real)v;
+      MW_HIP(hipMemcpyAsync(red, slots.data(), sizeof(real) * slots.size(), hipMemcpyHostToDevice, stream));
+    }
     else if (name == "n_leaves") s.n_leaves = (int)v;
     else if (name == "first") s.first = (int)v;
     else if (name == "mx_eta") s.mx[0] = (real)v;

@@ -9,6 +9,7 @@ constexpr int WG = 256;          // work-group size (4 wave64s)
 constexpr int EMIT_ITEMS = 4;    // level-(L-1) nodes per thread in leaf emission
 constexpr int EMIT_TILE = WG * EMIT_ITEMS;
 constexpr int MAX_SRC = 4096;
+constexpr int RED_SLOTS = 4096;  // max work-groups of a leaf kernel (one reduction slot each)
 
 // Scalars that live on the device for the whole run, so that a step needs no
 // host round trip (the reference reads dt, N_a and the maxima back every step).
@@ -16,8 +17,6 @@ template <typename real>
 struct DevState {
   real time, dt, sim_time;
   real mx[5];                     // eta, h, qx, qy(=qx), z used by this step's encode
-  unsigned long long dt_acc;      // running min of the next dt, as real bits
-  unsigned long long mx_acc[4];   // running max |eta|, |h|, |qx|, |z|, as real bits
   int n_leaves;
   int done;                       // set once time >= sim_time at the top of a step
   int first;                      // 1 until the first step's prologue ran
@@ -25,11 +24,7 @@ struct DevState {
   real inlet[4];                  // N, E, S, W boundary inlet values
   int row[4];                     // time-series cursors
   int n_src;
-  // work counters summed over steps (for the algorithmic-byte model)
-  unsigned long long leaf_sum;    // leaves updated
-  unsigned long long pe_sum;      // parents encoded (previous tree)
-  unsigned long long pd_sum;      // parents decoded (new tree)
-  unsigned long long ps_sum;      // parents re-encoded between RK stages
+  unsigned long long leaf_sum;    // leaves updated, summed over steps
 };
 
 // Time series of the boundaries (entries 0..3) and point sources (4..).
