@@ -135,6 +135,16 @@ def main():
                 "alg_bytes_per_launch": int(kern[dom]["alg_bytes_per_launch"]),
                 "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
 
+    copy_gbs = mwdg2.stream_copy_gbs(1 << 30, 20)
+    if roof:
+        roof["measured_copy_gbs"] = round(copy_gbs, 1)
+        roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbs, 4)
+    if args.profile_all:
+        out_k = {}
+        for name, (n, ms) in sorted(prof.items()):
+            out_k[name] = {"launches": n, "avg_us": round(1000.0 * ms / n, 2), "ms_per_step": round(ms / max(1, steps), 4)}
+        print(json.dumps({"per_kernel": out_k}), file=sys.stderr)
+
     cpu = None
     if not args.no_cpu:
         sim.close()

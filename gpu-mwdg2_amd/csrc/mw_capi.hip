@@ -434,10 +434,11 @@ struct Ctx : CtxBase {
       hipLaunchKernelGGL((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx);
     });
   }
+  int top_level() const { return std::min(TOP_LEVEL, L - 1); }
   void encode_launch() {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
-    for (int level = L - 1; level >= 0; --level) {
+    for (int level = L - 1; level > top_level(); --level) {
       const long long n = 1LL << (2 * level);
       timed("encode", [&] {
         if (mwm)
@@ -448,11 +449,27 @@ struct Ctx : CtxBase {
                              preflag, level, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       });
     }
+    timed("encode_top", [&] {
+      if (mwm)
+        hipLaunchKernelGGL((k_encode_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                           preflag, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0]);
+      else
+        hipLaunchKernelGGL((k_encode_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                           preflag, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0]);
+    });
   }
   void decode_launch() {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
-    for (int level = 0; level < L; ++level) {
+    timed("decode_top", [&] {
+      if (mwm)
+        hipLaunchKernelGGL((k_topdown_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                           leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
+      else
+        hipLaunchKernelGGL((k_topdown_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+                           leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
+    });
+    for (int level = top_level() + 1; level < L; ++level) {
       const long long n = 1LL << (2 * level);
       timed("decode", [&] {
         if (mwm)
@@ -476,7 +493,7 @@ struct Ctx : CtxBase {
                          nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
     });
   }
-  int leaf_grid() const { return std::min(nblk(N), RED_SLOTS); }
+  int leaf_grid() const { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
   void stage1_launch() {
     const int other = main_h ^ 1;
     timed(mwm ? "stage1" : "fv1", [&] {
@@ -495,12 +512,16 @@ struct Ctx : CtxBase {
     if (!mwm) main_h = other;
   }
   void encode_scale_launch() {
-    for (int level = L - 1; level >= 0; --level) {
+    for (int level = L - 1; level > top_level(); --level) {
       const long long n = 1LL << (2 * level);
       timed("encode_scale", [&] {
         hipLaunchKernelGGL((k_encode_scale<real>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[1], sig[cur], level, cnt[2]);
       });
     }
+    timed("encode_scale_top", [&] {
+      hipLaunchKernelGGL((k_encode_scale_top<real>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[1], sig[cur], top_level(),
+                         cnt[2]);
+    });
   }
   void stage2_launch() {
     if (hc.limitslopes) {
@@ -903,6 +924,32 @@ int mw_sort_pairs(int n, const int* keys, const int* vals, int* keys_out, int* v
     tmp.sort_pairs(n, k0, v0, k1, v1);
     MW_HIP(hipMemcpy(keys_out, k0, sizeof(int) * n, hipMemcpyDeviceToHost));
     MW_HIP(hipMemcpy(vals_out, v0, sizeof(int) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int mw_stream_copy(long long bytes, int reps, double* gbs) {
+  MW_NEED(bytes >= 16 && reps > 0 && gbs);
+  return guarded([&] {
+    const long long n = bytes / 16;
+    mw::Ctx<float> tmp;
+    MW_HIP(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+    float4* a = tmp.alloc<float4>(n);
+    float4* b = tmp.alloc<float4>(n);
+    MW_HIP(hipMemsetAsync(a, 0, n * 16, tmp.stream));
+    const int grid = 256 * 32;
+    hipLaunchKernelGGL(mw::k_stream_copy, dim3(grid), dim3(mw::WG), 0, tmp.stream, a, b, n);   // warm-up
+    hipEvent_t e0, e1;
+    MW_HIP(hipEventCreate(&e0));
+    MW_HIP(hipEventCreate(&e1));
+    MW_HIP(hipEventRecord(e0, tmp.stream));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(mw::k_stream_copy, dim3(grid), dim3(mw::WG), 0, tmp.stream, a, b, n);
+    MW_HIP(hipEventRecord(e1, tmp.stream));
+    MW_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
   });
 }
 

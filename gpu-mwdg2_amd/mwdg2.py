@@ -78,6 +78,7 @@ def lib():
         L.mw_profile.argtypes = [vp, i]
         L.mw_profile_read.argtypes = [vp, ctypes.POINTER(KernelTime), i, ctypes.POINTER(i)]
         L.mw_filter_table.argtypes = [i, vp]
+        L.mw_stream_copy.argtypes = [ll, i, ctypes.POINTER(d)]
         L.mw_run_simulation.argtypes = [i, ctypes.POINTER(cp)]
         _lib = L
     return _lib
@@ -186,6 +187,13 @@ def sort_pairs(keys, vals):
     vo = np.empty_like(v)
     _check(lib().mw_sort_pairs(k.size, k.ctypes.data, v.ctypes.data, ko.ctypes.data, vo.ctypes.data))
     return ko, vo
+
+
+def stream_copy_gbs(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Measured copy bandwidth (read + write bytes per second) in GB/s."""
+    g = ctypes.c_double()
+    _check(lib().mw_stream_copy(int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 def filter_table(precision: int = 8) -> np.ndarray:

@@ -137,6 +137,10 @@ int mw_set_scalar(mw_ctx* ctx, const char* name, double v);
  * done on the GPU (cub::DeviceRadixSort::SortPairs, get_sorting_indices.cu:24-57). */
 int mw_sort_pairs(int n, const int* keys, const int* vals, int* keys_out, int* vals_out);
 
+/* Achievable HBM bandwidth: a plain copy of `bytes` bytes repeated `reps`
+ * times; *gbs = (read + write bytes) / time. No reference counterpart. */
+int mw_stream_copy(long long bytes, int reps, double* gbs);
+
 /* Per-kernel timing with HIP events on the context stream: enable = 0 off,
  * 1 every kernel, 2 only the cell-update kernels (stage1, stage2, fv1). */
 int mw_profile(mw_ctx* ctx, int enable);
