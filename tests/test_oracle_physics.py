@@ -1,0 +1,199 @@
+"""Pins the oracle's operators against textbook results and invariants fixed
+by the mathematics, not against a restatement of its own formulas.
+
+- HLL flux (operators/flux_HLL.cuh:9-111): Toro's HLL with two-rarefaction
+  wave-speed estimates, dry-bed speeds and the transverse velocity upwinded by
+  the middle-wave speed (Toro, Shock-Capturing Methods for Free-Surface
+  Shallow Flows, 2001, ch. 10), written here from the book; consistency
+  F(U, U) = F(U) and rotational symmetry x <-> y.
+- Friction (operators/apply_friction.cuh:7-53): Manning's law as dt -> 0 and
+  the closed-form large-dt limit of the linearised implicit update.
+- Well-balancedness: a lake at rest over a rough bed stays at rest (FP64,
+  |q| <= 1e-10), both solvers (the reference's c-property check,
+  cases/tests/test.py:105,396).
+- Mass: closed walls, no friction, no sources, uniform grid (epsilon 0)
+  conserve sum(h0 * dx^2). On an adaptive grid the reference's neighbour
+  rule (a leaf sees the neighbour at its own level, find_neighbours.cu:87-108)
+  makes the fluxes on either side of a resolution change differ, so mass is
+  only approximately conserved there and no invariant is claimed.
+- CFL: dt_next equals the exact minimum of the per-leaf CFL step.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+
+G = 9.80665
+TOL = 1e-3
+
+
+def toro_hll_x(UL, UR, g=G, tol=TOL):
+    hL, huL, hvL = UL
+    hR, huR, hvR = UR
+    if hL < tol and hR < tol:
+        return np.zeros(3)
+    uL, vL = (huL / hL, hvL / hL) if hL >= tol else (0.0, 0.0)
+    uR, vR = (huR / hR, hvR / hR) if hR >= tol else (0.0, 0.0)
+    hL_, hR_ = (hL if hL >= tol else 0.0), (hR if hR >= tol else 0.0)
+    aL, aR = math.sqrt(g * hL_), math.sqrt(g * hR_)
+    # two-rarefaction estimates of the star region
+    h_star = (0.5 * (aL + aR) + 0.25 * (uL - uR)) ** 2 / g
+    u_star = 0.5 * (uL + uR) + aL - aR
+    a_star = math.sqrt(g * h_star)
+    sL = uR - 2 * aR if hL_ < tol else min(uL - aL, u_star - a_star)
+    sR = uL + 2 * aL if hR_ < tol else max(uR + aR, u_star + a_star)
+    FL = np.array([hL_ * uL, hL_ * uL * uL + 0.5 * g * hL_ ** 2, hL_ * uL * vL])
+    FR = np.array([hR_ * uR, hR_ * uR * uR + 0.5 * g * hR_ ** 2, hR_ * uR * vR])
+    if sL >= 0:
+        return FL
+    if sR < 0:
+        return FR
+    UL_ = np.array([hL_, hL_ * uL]); UR_ = np.array([hR_, hR_ * uR])
+    F = (sR * FL[:2] - sL * FR[:2] + sL * sR * (UR_ - UL_)) / (sR - sL)
+    s_mid = (sL * hR_ * (uR - sR) - sR * hL_ * (uL - sL)) / (hR_ * (uR - sR) - hL_ * (uL - sL))
+    return np.array([F[0], F[1], F[0] * (vL if s_mid >= 0 else vR)])
+
+
+STATES = [
+    ((2.0, 0.0, 0.0), (0.5, 0.0, 0.0)),          # dam break, still water
+    ((1.0, 2.5, 0.3), (1.2, -0.4, -0.7)),         # converging, transverse flow
+    ((1.5, 6.0, 1.0), (1.4, 5.0, -1.0)),          # supercritical to the right
+    ((1.0, -5.0, 0.2), (1.1, -6.0, 0.4)),         # supercritical to the left
+    ((0.0, 0.0, 0.0), (1.0, -0.5, 0.25)),         # dry left bed
+    ((1.0, 0.8, -0.3), (0.0, 0.0, 0.0)),          # dry right bed
+    ((0.3, -0.2, 0.1), (0.4, 0.1, 0.0)),          # shallow rarefaction
+]
+
+
+@pytest.mark.parametrize("UL,UR", STATES)
+def test_hll_matches_textbook(UL, UR):
+    got = oracle.flux_hll(UL, UR, axis="x")
+    exp = toro_hll_x(UL, UR)
+    assert np.allclose(got, exp, rtol=1e-13, atol=1e-13), (got, exp)
+
+
+@pytest.mark.parametrize("U", [(1.0, 0.5, -0.2), (2.0, 0.0, 0.0), (0.7, -3.0, 1.0)])
+def test_hll_consistency(U):
+    h, hu, hv = U
+    phys = np.array([hu, hu * hu / h + 0.5 * G * h * h, hu * hv / h])
+    assert np.allclose(oracle.flux_hll(U, U, axis="x"), phys, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("UL,UR", STATES)
+def test_hll_y_is_rotated_x(UL, UR):
+    # y flux of (h, qx, qy) equals the x flux of (h, qy, -qx), rotated back
+    rl = (UL[0], UL[2], -UL[1]); rr = (UR[0], UR[2], -UR[1])
+    fx = oracle.flux_hll(rl, rr, axis="x")
+    fy = oracle.flux_hll(UL, UR, axis="y")
+    assert np.allclose(fy, [fx[0], -fx[2], fx[1]], rtol=1e-14, atol=1e-14)
+
+
+def test_friction_tends_to_manning_law():
+    h, qx, qy, n = 0.8, 0.6, -0.25, 0.03
+    u, v = qx / h, qy / h
+    s = math.hypot(u, v)
+    # Manning: dq/dt = -g n^2 q |q| / h^(7/3)
+    rate = np.array([-G * n * n * u * s / h ** (1 / 3), -G * n * n * v * s / h ** (1 / 3)])
+    for dt in (1e-4, 1e-5, 1e-6):
+        q = oracle.friction(h, qx, qy, n, dt)
+        est = (q - [qx, qy]) / dt
+        assert np.allclose(est, rate, rtol=50 * dt, atol=0)
+
+
+def test_friction_large_dt_limit():
+    # the linearised implicit step tends to q_x u^2 / (2u^2 + v^2) as dt -> inf
+    h, qx, qy, n = 0.5, 0.4, 0.3, 0.05
+    u, v = qx / h, qy / h
+    q = oracle.friction(h, qx, qy, n, 1e14)
+    assert np.allclose(q, [qx * u * u / (2 * u * u + v * v), qy * v * v / (2 * v * v + u * u)], rtol=1e-9)
+    q1 = oracle.friction(h, qx, 0.0, n, 1e14)
+    assert abs(q1[0] - qx / 2) < 1e-9 and q1[1] == 0.0
+
+
+def test_friction_below_thresholds_zeroes_discharge():
+    assert np.array_equal(oracle.friction(5e-4, 0.3, 0.2, 0.03, 0.1), [0.0, 0.0])
+    assert np.array_equal(oracle.friction(1.0, 1e-10, -1e-10, 0.03, 0.1), [0.0, 0.0])
+
+
+def lake_case(d, L, solver, eta, seed=4, manning=0.0, t_end=5.0):
+    os.makedirs(d, exist_ok=True)
+    n = 1 << L
+    z = cases.gaussian_random_field(n, seed, 0.5) + 1.0
+    h = np.maximum(0.0, eta - z)
+    cases.write_raster(os.path.join(d, "case.dem"), z, 0.0, 0.0, 1.0, fmt="%.17g")
+    cases.write_raster(os.path.join(d, "case.start"), h, 0.0, 0.0, 1.0, fmt="%.17g")
+    par = os.path.join(d, "case.par")
+    cases.write_par(par, cases._par(solver, L, 1e-3, t_end, manning))
+    return par, z
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_lake_at_rest_fully_wet(tmp_path, solver):
+    par, z = lake_case(str(tmp_path / solver), 5, solver, eta=10.0)
+    o = oracle.Oracle(par, "f64")
+    try:
+        for _ in range(8):
+            o.step()
+        n = int(o.scalar("length"))
+        for m in (("0", "1x", "1y") if solver == "mwdg2" else ("0",)):
+            assert np.max(np.abs(o.get(f"assem.qx{m}")[:n])) <= 1e-10
+            assert np.max(np.abs(o.get(f"assem.qy{m}")[:n])) <= 1e-10
+        eta0 = o.get("assem.h0")[:n] + o.get("assem.z0")[:n]
+        assert np.max(np.abs(eta0 - 10.0)) <= 1e-10
+    finally:
+        o.close()
+
+
+def total_mass(o):
+    n = int(o.scalar("length"))
+    L = int(o.scalar("L"))
+    lev = o.get("assem.lev")[:n]
+    h0 = o.get("assem.h0")[:n]
+    dx = o.scalar("dx")
+    return float(np.sum(h0 * (dx * (2.0 ** (L - lev))) ** 2))
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_mass_conserved_with_closed_walls(tmp_path, solver):
+    par = cases.circular_dam_break(str(tmp_path / solver), L=6, eps=0.0, t_end=2.0, solver=solver,
+                                   cfl=0.33 if solver == "mwdg2" else None)
+    o = oracle.Oracle(par, "f64")
+    try:
+        o.step()
+        m0 = total_mass(o)
+        for _ in range(10):
+            o.step()
+        assert int(o.scalar("length")) == 4 ** 6
+        # telescoping fluxes: each face's flux leaves one cell and enters the other
+        assert abs(total_mass(o) - m0) <= 1e-12 * m0
+    finally:
+        o.close()
+
+
+def test_cfl_min_is_exact_minimum(tmp_path):
+    par = cases.random_case(str(tmp_path / "cfl"), L=5, seed=2, t_end=1.0)
+    o = oracle.Oracle(par, "f64")
+    try:
+        for _ in range(3):
+            o.step()
+        n = int(o.scalar("length"))
+        L = int(o.scalar("L"))
+        lev = o.get("assem.lev")[:n]
+        h = o.get("assem.h0")[:n]
+        qx = o.get("assem.qx0")[:n]
+        qy = o.get("assem.qy0")[:n]
+        dxl = o.scalar("dx") * 2.0 ** (L - lev)
+        cfl = o.scalar("CFL")
+        wet = h >= TOL
+        with np.errstate(divide="ignore", invalid="ignore"):
+            c = np.sqrt(G * h)
+            dtx = cfl * dxl / (np.abs(qx / h) + c)
+            dty = cfl * dxl / (np.abs(qy / h) + c)
+        cand = np.where(wet, np.minimum(dtx, dty), o.scalar("initial_tstep"))
+        assert o.scalar("dt") == pytest.approx(float(cand.min()), rel=1e-14)
+    finally:
+        o.close()
