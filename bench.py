@@ -48,16 +48,26 @@ def make_case(L: int) -> str:
     return par
 
 
-def stage_bytes(real_bytes: int, leaves: int) -> dict:
-    """Algorithmic HBM bytes of the update kernels for `leaves` leaf updates.
+def kernel_bytes(s: int, leaves: int, wet1: int, wet2: int) -> dict:
+    """Algorithmic HBM bytes of the update kernels over the timed steps.
 
-    Each logical element the stage must touch is counted once (neighbour
-    re-reads are not): stage 1 reads the leaf's 12 coefficients (eta, qx, qy,
-    z x {0,1x,1y}) and writes 9; stage 2 reads the 12 stage-1 coefficients and
-    the 9 stage-start coefficients, writes 9, and reduces one dt per leaf.
+    Every logical element a kernel must touch is counted once (neighbour
+    re-reads are not). A leaf holds 12 coefficients (eta, qx, qy, z x
+    {0, 1x, 1y}); 9 of them (not z) are written. Stage 1: the sweep reads all
+    leaves and writes the dry ones (and a 4-byte list entry per wet leaf); the
+    wet kernel reads and writes the wet leaves. Stage 2 likewise, where a leaf
+    also reads its 9 stage-start coefficients from A.
     """
-    s = real_bytes
-    return {"stage1": leaves * (12 * s + 9 * s), "stage2": leaves * (12 * s + 9 * s + 9 * s + s)}
+    i = 4
+    return {
+        "sweep1": leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * i,
+        "wet1": wet1 * (12 * s + 9 * s),
+        "sweep2": leaves * 12 * s + (leaves - wet2) * (9 * s + 9 * s) + wet2 * i,
+        "wet2": wet2 * (12 * s + 9 * s + 9 * s),
+        # one-pass kernels (HWFV1 and the limiter path)
+        "stage1": leaves * (12 * s + 9 * s),
+        "stage2": leaves * (12 * s + 9 * s + 9 * s + s),
+    }
 
 
 def run_cpu_baseline(par: str, L: int, budget_s: float) -> dict:
@@ -118,10 +128,10 @@ def main():
     value = (4 ** args.L) * steps / wall
 
     leaves = c1["leaf_updates"] - c0["leaf_updates"]
-    rb = 4
-    alg = stage_bytes(rb, leaves)
+    wet = [c1["wet_updates"][q] - c0["wet_updates"][q] for q in range(2)]
+    alg = kernel_bytes(4, leaves, wet[0], wet[1])
     kern = {}
-    for k in ("stage1", "stage2"):
+    for k in alg:
         if k in prof:
             n, ms = prof[k]
             kern[k] = {"launches": n, "avg_ms": ms / n, "alg_bytes_per_launch": alg[k] / n,
@@ -161,6 +171,7 @@ def main():
                    "mean_leaves": leaves / max(1, steps),
                    "mean_parents_encoded": (c1["parents_encoded"] - c0["parents_encoded"]) / max(1, steps),
                    "mean_parents_decoded": (c1["parents_decoded"] - c0["parents_decoded"]) / max(1, steps),
+                   "mean_wet_leaves": [w / max(1, steps) for w in wet],
                    "sim_time_reached": c1["time"], "setup_s": round(t_setup, 2),
                    "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                for k, v in kern.items()}},

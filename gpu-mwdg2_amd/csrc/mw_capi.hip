@@ -77,7 +77,8 @@ struct CtxBase {
   }
   template <typename F>
   void timed(const char* name, F&& launch) {
-    const bool update = !std::strncmp(name, "stage", 5) || !std::strncmp(name, "fv1", 3);
+    const bool update = !std::strncmp(name, "stage", 5) || !std::strncmp(name, "fv1", 3) ||
+                        !std::strncmp(name, "sweep", 5) || !std::strncmp(name, "wet", 3);
     if (!profiling || (profiling == 2 && !update)) { launch(); return; }
     hipEvent_t a = ev(), b = ev();
     MW_HIP(hipEventRecord(a, stream));
@@ -129,7 +130,8 @@ struct Ctx : CtxBase {
   int *rev_z = nullptr, *rev_rm = nullptr;
   real* uprev = nullptr;
   unsigned long long* maxh2 = nullptr;
-  real* red = nullptr;                        // [5][RED_SLOTS] next-step reduction slots
+  real* red = nullptr;                        // [5][2 * leaf_grid()] next-step reduction slots
+  WetList wet[2]{};                           // wet-leaf lists of the two MWDG2 stages
   unsigned long long* cnt[3] = {nullptr, nullptr, nullptr};  // P_e, P_d, rescaled parents (slot arrays)
   // series and sources on the device
   int *s_off = nullptr, *s_len = nullptr;
@@ -194,7 +196,29 @@ struct Ctx : CtxBase {
     rev_rm = alloc<int>(N);
     st = alloc<DevState<real>>(1);
     maxh2 = alloc<unsigned long long>(1);
-    red = alloc<real>((size_t)5 * RED_SLOTS);
+    red = alloc<real>((size_t)5 * 2 * RED_SLOTS);
+    {
+      // neutral slots (dt +inf, maxima 0): kernels that do not write a slot leave it neutral
+      // layout [5][2 * leaf_grid()]: row 0 holds dt candidates, rows 1-4 maxima
+      std::vector<real> init((size_t)5 * 2 * RED_SLOTS, real(0));
+      for (int i = 0; i < 2 * leaf_grid(); ++i) init[i] = real(__builtin_inff());
+      MW_HIP(hipMemcpyAsync(red, init.data(), sizeof(real) * init.size(), hipMemcpyHostToDevice, stream));
+      MW_HIP(hipStreamSynchronize(stream));
+    }
+    if (mwm && !hc.limitslopes) {
+      const int G = leaf_grid(), per = G / 8;
+      const long long tmax = nblk(N);
+      const int cap_tiles = (int)(((tmax + 7) / 8 + per - 1) / per) + 1;
+      for (int q = 0; q < 2; ++q) {
+        wet[q].cap = cap_tiles * WG;
+        wet[q].raw = alloc<int>((size_t)G * wet[q].cap);
+        wet[q].cnt = alloc<int>(G);
+        wet[q].off = alloc<int>(G + 1);
+        wet[q].list = alloc<int>(N);
+        wet[q].sum = &st->wet_sum[q];
+        MW_HIP(hipMemsetAsync(wet[q].off, 0, sizeof(int) * (G + 1), stream));
+      }
+    }
     for (int q = 0; q < 3; ++q) {
       cnt[q] = alloc<unsigned long long>(CNT_SLOTS);
       MW_HIP(hipMemsetAsync(cnt[q], 0, sizeof(unsigned long long) * CNT_SLOTS, stream));
@@ -425,7 +449,8 @@ struct Ctx : CtxBase {
   // ---- step pieces ---------------------------------------------------------
   void k_prologue_launch() {
     timed("prologue", [&] {
-      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(WG), 0, stream, st, P, series(), sources(), red, leaf_grid());
+      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(WG), 0, stream, st, P, series(), sources(), red,
+                         2 * leaf_grid());
     });
   }
   void k_sources_launch() {
@@ -494,20 +519,35 @@ struct Ctx : CtxBase {
     });
   }
   int leaf_grid() const { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
+  void wet_list_launch(int q) {
+    const int G = leaf_grid();
+    timed("wet_list", [&] {
+      hipLaunchKernelGGL(k_wet_scan, dim3(1), dim3(1024), 0, stream, &st->done, wet[q], G);
+      hipLaunchKernelGGL(k_wet_gather, dim3(G), dim3(WG), 0, stream, &st->done, wet[q]);
+    });
+  }
   void stage1_launch() {
     const int other = main_h ^ 1;
+    const int G = leaf_grid();
+    if (mwm && !hc.limitslopes) {
+      timed("sweep1", [&] {
+        hipLaunchKernelGGL((k_sweep1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other],
+                           zh, wet[0]);
+      });
+      wet_list_launch(0);
+      timed("wet1", [&] {
+        hipLaunchKernelGGL((k_wet1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other], zh,
+                           wet[0], G);
+      });
+      return;
+    }
     timed(mwm ? "stage1" : "fv1", [&] {
-      if (mwm) {
-        if (hc.limitslopes)
-          hipLaunchKernelGGL((k_stage1<real, true, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
-                             hier[main_h], hier[other], zh, uprev, red);
-        else
-          hipLaunchKernelGGL((k_stage1<real, true, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
-                             hier[main_h], hier[other], zh, uprev, red);
-      } else {
-        hipLaunchKernelGGL((k_stage1<real, false, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(),
+      if (mwm)
+        hipLaunchKernelGGL((k_stage1<real, true, true>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h],
+                           hier[other], zh, uprev, red);
+      else
+        hipLaunchKernelGGL((k_stage1<real, false, false>), dim3(G), dim3(WG), 0, stream, st, P, leaves(),
                            hier[main_h], hier[other], zh, uprev, red);
-      }
     });
     if (!mwm) main_h = other;
   }
@@ -524,21 +564,27 @@ struct Ctx : CtxBase {
     });
   }
   void stage2_launch() {
+    const int G = leaf_grid();
     if (hc.limitslopes) {
       MW_HIP(hipMemsetAsync(maxh2, 0, sizeof(unsigned long long), stream));
       timed("maxh2", [&] {
-        hipLaunchKernelGGL((k_maxh_stage2<real>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[1], zh, maxh2);
+        hipLaunchKernelGGL((k_maxh_stage2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[1], zh, maxh2);
       });
       timed("stage2", [&] {
-        hipLaunchKernelGGL((k_stage2<real, true>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
+        hipLaunchKernelGGL((k_stage2_limited<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
                            zh, uprev, maxh2, red);
       });
-    } else {
-      timed("stage2", [&] {
-        hipLaunchKernelGGL((k_stage2<real, false>), dim3(leaf_grid()), dim3(WG), 0, stream, st, P, leaves(), hier[0],
-                           hier[1], zh, uprev, maxh2, red);
-      });
+      return;
     }
+    timed("sweep2", [&] {
+      hipLaunchKernelGGL((k_sweep2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1],
+                         red);
+    });
+    wet_list_launch(1);
+    timed("wet2", [&] {
+      hipLaunchKernelGGL((k_wet2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1], G,
+                         red);
+    });
   }
 
   void rebuild() {
@@ -575,7 +621,7 @@ struct Ctx : CtxBase {
 
   // minimum over the reduction slots written by the last update kernel
   real next_dt() {
-    std::vector<real> v(leaf_grid());
+    std::vector<real> v(2 * leaf_grid());
     MW_HIP(hipMemcpyAsync(v.data(), red, sizeof(real) * v.size(), hipMemcpyDeviceToHost, stream));
     MW_HIP(hipStreamSynchronize(stream));
     real m = v[0];
@@ -608,6 +654,8 @@ struct Ctx : CtxBase {
     c->parents_encoded = (long long)slot_sum(cnt[0]);
     c->parents_decoded = (long long)slot_sum(cnt[1]);
     c->parents_rescaled = (long long)slot_sum(cnt[2]);
+    c->wet_updates[0] = (long long)s.wet_sum[0];
+    c->wet_updates[1] = (long long)s.wet_sum[1];
   }
 
   void stage(const std::string& s) override {
@@ -665,7 +713,7 @@ struct Ctx : CtxBase {
     if (name == "time") s.time = (real)v;
     else if (name == "dt") s.dt = (real)v;
     else if (name == "dt_next") {
-      std::vector<real> slots(leaf_grid(), real(__builtin_inff()));
+      std::vector<real> slots(2 * leaf_grid(), real(__builtin_inff()));
       slots[0] = (real)v;
       MW_HIP(hipMemcpyAsync(red, slots.data(), sizeof(real) * slots.size(), hipMemcpyHostToDevice, stream));
     }

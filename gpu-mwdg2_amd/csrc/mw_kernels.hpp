@@ -25,6 +25,7 @@ struct DevState {
   int row[4];                     // time-series cursors
   int n_src;
   unsigned long long leaf_sum;    // leaves updated, summed over steps
+  unsigned long long wet_sum[2];  // leaves of the wet lists of stage 1 and 2, summed over steps
 };
 
 // Time series of the boundaries (entries 0..3) and point sources (4..).

@@ -34,11 +34,13 @@ class Counters(ctypes.Structure):
                 ("done", ctypes.c_int), ("max_ref_lvl", ctypes.c_int), ("solver", ctypes.c_int),
                 ("precision", ctypes.c_int), ("maxes", ctypes.c_double * 5),
                 ("leaf_updates", ctypes.c_longlong), ("parents_encoded", ctypes.c_longlong),
-                ("parents_decoded", ctypes.c_longlong), ("parents_rescaled", ctypes.c_longlong)]
+                ("parents_decoded", ctypes.c_longlong), ("parents_rescaled", ctypes.c_longlong),
+                ("wet_updates", ctypes.c_longlong * 2)]
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "maxes"}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("maxes", "wet_updates")}
         d["maxes"] = list(self.maxes)
+        d["wet_updates"] = list(self.wet_updates)
         return d
 
 

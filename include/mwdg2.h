@@ -50,6 +50,8 @@ typedef struct mw_counters {
   long long parents_encoded;     /* parents of the previous tree encoded (P_e)      */
   long long parents_decoded;     /* significant parents decoded (P_d)               */
   long long parents_rescaled;    /* parents re-encoded between the RK stages        */
+  long long wet_updates[2];      /* leaves given the full update in RK stage 1 / 2;
+                                    the others were dry with dry neighbours         */
 } mw_counters;
 
 typedef struct mw_boundary {     /* classes/Boundary.h:17-31, one side            */

@@ -54,3 +54,55 @@ def test_epsilon_zero(case_dir):
 def test_fp32_free_run_urban(case_dir):
     par = _mk(case_dir, "urban8", cases.urban, L=8, t_end=2.0)
     parity_util.run_pair(par, "f32", steps=10)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_slope_limiter(case_dir, prec):
+    par = _mk(case_dir, f"lim_{prec}", cases.random_case, L=6, seed=17, manning=0.02, t_end=1.0, startq=True)
+    with open(par, "a") as f:
+        f.write("limitslopes\ntol_Krivo     1\n")
+    parity_util.run_pair(par, prec, steps=8)
+
+
+def test_urban_l10_wet_lists(case_dir):
+    """Thousands of work-groups with wet leaves spread over many of them."""
+    par = _mk(case_dir, "urban10", cases.urban, L=10, t_end=100.0)
+    reps = parity_util.run_pair(par, "f32", steps=4)
+    assert reps[-1]["n_leaves"][0] > 100000
+
+
+def test_hwfv1_urban_l9(case_dir):
+    par = _mk(case_dir, "urban9hw", cases.urban, L=9, t_end=100.0, solver="hwfv1")
+    parity_util.run_pair(par, "f32", steps=4)
+
+
+@pytest.mark.parametrize("n,span", [(1, 10), (1000, 50), ((1 << 20) + 123, 1 << 30), (300000, 7)])
+def test_radix_sort_pairs(n, span):
+    """Stable LSD radix sort against numpy's stable argsort (duplicates included)."""
+    import mwdg2
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, span, size=n, dtype=np.int64).astype(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    ko, vo = mwdg2.sort_pairs(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ko, keys[order])
+    assert np.array_equal(vo, vals[order])
+
+
+def test_run_until_end_time(case_dir):
+    """The device-resident loop stops at sim_time with the last step clipped
+    (run_simulation.cu:228-235), as the oracle does."""
+    import mwdg2
+    import oracle
+    par = _mk(case_dir, "cdb_end", cases.circular_dam_break, L=5, t_end=0.05)
+    o = oracle.Oracle(par, "f64")
+    n = 0
+    while o.step():
+        n += 1
+    t_o = o.scalar("time")
+    o.close()
+    with mwdg2.Simulation(par, "f64") as s:
+        c = s.run(max_steps=10000, batch=7)
+    assert c["done"] == 1
+    assert c["time"] == t_o and abs(t_o - 0.05) < 1e-12
+    assert c["steps"] == n
