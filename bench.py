@@ -70,6 +70,22 @@ def kernel_bytes(s: int, leaves: int, wet1: int, wet2: int) -> dict:
     }
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` (read + write) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, each counter
+    scaled by its reading for a 1 GiB copy kernel; None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "update_mra_kernels_urban_L12.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1] == f"k_{kernel}<float>":
+                    return int(float(parts[2]) + float(parts[3]))
+    except OSError:
+        pass
+    return None
+
+
 def run_cpu_baseline(par: str, L: int, budget_s: float) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -141,7 +157,8 @@ def main():
     if dom:
         a = kern[dom]["gbs"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(a / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom) if args.L == 12 else None,
                 "alg_bytes_per_launch": int(kern[dom]["alg_bytes_per_launch"]),
                 "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
 
