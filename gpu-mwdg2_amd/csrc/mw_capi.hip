@@ -130,6 +130,8 @@ struct Ctx : CtxBase {
   int *rev_z = nullptr, *rev_rm = nullptr;
   real* uprev = nullptr;
   unsigned long long* maxh2 = nullptr;
+  size_t Hp = 0;                              // node count rounded up to whole sibling groups
+  real* xbuf = nullptr;                       // one coefficient as a plain array (parity transfers)
   real* red = nullptr;                        // [5][2 * leaf_grid()] next-step reduction slots
   WetList wet[2]{};                           // wet-leaf lists of the two MWDG2 stages
   unsigned long long* cnt[3] = {nullptr, nullptr, nullptr};  // P_e, P_d, rescaled parents (slot arrays)
@@ -171,18 +173,15 @@ struct Ctx : CtxBase {
     H = lvl_idx(L + 1);
     mwm = hc.solver == MWDG2;
     MW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    const int nf = mwm ? 9 : 3;
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 9; ++v) {
-        const bool used = mwm || (v % 3 == 0);
-        hier[b].f[v] = used ? alloc<real>(H) : nullptr;
-        if (used) MW_HIP(hipMemsetAsync(hier[b].f[v], 0, sizeof(real) * H, stream));
-      }
-    (void)nf;
-    for (int m = 0; m < 3; ++m) {
-      zh.z[m] = (mwm || m == 0) ? alloc<real>(H) : nullptr;
-      if (zh.z[m]) MW_HIP(hipMemsetAsync(zh.z[m], 0, sizeof(real) * H, stream));
+    Hp = ((size_t)H + 3) / 4 * 4;
+    for (int b = 0; b < 2; ++b) {
+      hier[b].eta0 = alloc<real>(Hp);
+      hier[b].rec = alloc<real>(Hp * (mwm ? REC_SLOTS<true> : REC_SLOTS<false>));
     }
+    zh.z0 = alloc<real>(Hp);
+    zh.rec = mwm ? alloc<real>(Hp * 2) : nullptr;
+    xbuf = alloc<real>(Hp);
+    clear_hierarchies();
     sig[0] = alloc<uint8_t>(D + 4);
     sig[1] = alloc<uint8_t>(D + 4);
     xflag = alloc<uint8_t>(D + 4);
@@ -227,6 +226,16 @@ struct Ctx : CtxBase {
     fill_params();
     upload_sources_and_series();
     build_initial();
+  }
+
+  void clear_hierarchies() {
+    const size_t rs = mwm ? REC_SLOTS<true> : REC_SLOTS<false>;
+    for (int b = 0; b < 2; ++b) {
+      MW_HIP(hipMemsetAsync(hier[b].eta0, 0, sizeof(real) * Hp, stream));
+      MW_HIP(hipMemsetAsync(hier[b].rec, 0, sizeof(real) * Hp * rs, stream));
+    }
+    MW_HIP(hipMemsetAsync(zh.z0, 0, sizeof(real) * Hp, stream));
+    if (zh.rec) MW_HIP(hipMemsetAsync(zh.rec, 0, sizeof(real) * Hp * 2, stream));
   }
 
   void fill_params() {
@@ -320,7 +329,6 @@ struct Ctx : CtxBase {
 
   void build_initial() {
     zorder_build();
-    const int modes = mwm ? 3 : 1;
     // row-major modal values -> Z-ordered finest level
     std::vector<real*> dm(12, nullptr);
     for (int v = 0; v < 12; ++v)
@@ -332,8 +340,12 @@ struct Ctx : CtxBase {
       }
     real** dtab = dalloc<real*>(12);
     MW_HIP(hipMemcpyAsync(dtab, dm.data(), sizeof(real*) * 12, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL((k_set_finest<real>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab, rev_z,
-                       hier[0], zh, L, N, modes);
+    if (mwm)
+      hipLaunchKernelGGL((k_set_finest<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab, rev_z,
+                         hier[0], zh, L, N);
+    else
+      hipLaunchKernelGGL((k_set_finest<real, false>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab,
+                         rev_z, hier[0], zh, L, N);
     MW_HIP(hipGetLastError());
     MW_HIP(hipStreamSynchronize(stream));
     for (real* p : dm) if (p) (void)hipFree(p);
@@ -680,13 +692,30 @@ struct Ctx : CtxBase {
     long long len = 0;
     size_t esz = 0;
     const char* fn[9] = {"eta0", "eta1x", "eta1y", "qx0", "qx1x", "qx1y", "qy0", "qy1x", "qy1y"};
+    int coef_id = -1, hb = 0;                 // hierarchy coefficient through xbuf
     if (name.size() > 2 && (name[0] == 'A' || name[0] == 'B') && name[1] == '.') {
-      const int b = (name[0] == 'A') ? (mwm ? 0 : main_h) : (mwm ? 1 : main_h ^ 1);
+      hb = (name[0] == 'A') ? (mwm ? 0 : main_h) : (mwm ? 1 : main_h ^ 1);
       for (int v = 0; v < 9; ++v)
-        if (name.substr(2) == fn[v]) { dev = hier[b].f[v]; len = H; esz = sizeof(real); }
-    } else if (name == "z0" || name == "z1x" || name == "z1y") {
-      dev = zh.z[name == "z0" ? 0 : (name == "z1x" ? 1 : 2)]; len = H; esz = sizeof(real);
-    } else if (name == "sig") { dev = sig[cur]; len = D; esz = 1; }
+        if (name.substr(2) == fn[v] && (mwm || v % 3 == 0)) coef_id = v;
+    } else if (name == "z0" || (mwm && (name == "z1x" || name == "z1y"))) {
+      coef_id = 9 + (name == "z0" ? 0 : (name == "z1x" ? 1 : 2));
+    }
+    if (coef_id >= 0) {
+      dev = xbuf; len = H; esz = sizeof(real);
+      if (host && count > 0) {
+        const int n = (int)std::min<long long>(count, len);
+        if (up) {
+          MW_HIP(hipMemcpyAsync(xbuf, host, sizeof(real) * n, hipMemcpyHostToDevice, stream));
+          coef_io_launch(hb, coef_id, n, false);
+        } else {
+          coef_io_launch(hb, coef_id, n, true);
+          MW_HIP(hipMemcpyAsync(host, xbuf, sizeof(real) * n, hipMemcpyDeviceToHost, stream));
+        }
+        MW_HIP(hipStreamSynchronize(stream));
+      }
+      return len;
+    }
+    if (name == "sig") { dev = sig[cur]; len = D; esz = 1; }
     else if (name == "sig_next") { dev = sig[cur ^ 1]; len = D; esz = 1; }
     else if (name == "xflag") { dev = xflag; len = D; esz = 1; }
     else if (name == "leaflev") { dev = leaflev; len = D; esz = 1; }
@@ -706,6 +735,16 @@ struct Ctx : CtxBase {
       MW_HIP(hipStreamSynchronize(stream));
     }
     return len;
+  }
+
+  void coef_io_launch(int b, int which, int n, bool to_buf) {
+    if (mwm)
+      hipLaunchKernelGGL((k_coef_io<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
+                         to_buf ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_coef_io<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
+                         to_buf ? 1 : 0);
+    MW_HIP(hipGetLastError());
   }
 
   void set_scalar(const std::string& name, double v) override {
@@ -777,10 +816,7 @@ struct Ctx : CtxBase {
   }
   void rebuild_all() {
     MW_HIP(hipStreamSynchronize(stream));
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 9; ++v)
-        if (hier[b].f[v]) MW_HIP(hipMemsetAsync(hier[b].f[v], 0, sizeof(real) * H, stream));
-    for (int m = 0; m < 3; ++m) if (zh.z[m]) MW_HIP(hipMemsetAsync(zh.z[m], 0, sizeof(real) * H, stream));
+    clear_hierarchies();
     for (uint8_t* p : {sig[0], sig[1], xflag, leaflev, preflag}) MW_HIP(hipMemsetAsync(p, 0, D + 4, stream));
     fill_params();
     upload_sources_and_series();

@@ -48,15 +48,53 @@ struct Params {
   int bstart[4], bend[4];
 };
 
+// Flow scale coefficients of every node of the hierarchy, eta form. Coefficient
+// v = 3 * var + mode: eta0 eta1x eta1y qx0 qx1x qx1y qy0 qy1x qy1y.
+// eta0 is a plain array over the nodes (read alone by the wet/dry probes and
+// the point sources). The other coefficients of the 4 siblings of a group
+// (nodes 4g .. 4g+3, the children of one parent) share a record
+// rec[g][slot][4]: 8 slots for MWDG2 (eta1x ... qy1y: one 128-byte line in
+// FP32) and 2 for HWFV1 (qx0, qy0). A leaf's coefficients then sit in one line
+// instead of eight, and a parent's 4 children in one record.
 template <typename real>
-struct Hier {                     // flow scale coefficients, eta form
-  real* f[9];                     // eta0 eta1x eta1y qx0 qx1x qx1y qy0 qy1x qy1y
+struct Hier {
+  real* eta0;
+  real* rec;
 };
 
+// Bed coefficients: z0 a plain array, z1x and z1y in 2-slot sibling records
+// (MWDG2 only).
 template <typename real>
 struct ZHier {
-  real* z[3];                     // z0 z1x z1y
+  real* z0;
+  real* rec;
 };
+
+template <bool MWm>
+constexpr int REC_SLOTS = MWm ? 8 : 2;
+
+template <bool MWm>
+__host__ __device__ constexpr int rec_slot(int v) { return MWm ? v - 1 : (v == 3 ? 0 : 1); }
+
+// the 4 siblings' values of coefficient v in group g (contiguous, 16-byte aligned)
+template <typename real, bool MWm>
+__host__ __device__ __forceinline__ real* coef4(const Hier<real>& S, int v, int g) {
+  if (v == 0) return S.eta0 + 4 * (size_t)g;
+  return S.rec + (size_t)g * (4 * REC_SLOTS<MWm>) + rec_slot<MWm>(v) * 4;
+}
+template <typename real, bool MWm>
+__host__ __device__ __forceinline__ real& coef(const Hier<real>& S, int v, int idx) {
+  return coef4<real, MWm>(S, v, idx >> 2)[idx & 3];
+}
+template <typename real>
+__host__ __device__ __forceinline__ real* zcoef4(const ZHier<real>& Z, int m, int g) {
+  if (m == 0) return Z.z0 + 4 * (size_t)g;
+  return Z.rec + (size_t)g * 8 + (m - 1) * 4;
+}
+template <typename real>
+__host__ __device__ __forceinline__ real& zcoef(const ZHier<real>& Z, int m, int idx) {
+  return zcoef4<real>(Z, m, idx >> 2)[idx & 3];
+}
 
 template <typename real>
 struct Sources {
