@@ -86,6 +86,43 @@ def measured_traffic(kernel: str):
     return None
 
 
+def finest_fields(sim, L: int):
+    """Leaf cell means (h, qx, qy) repeated over the finest cells each leaf
+    covers, in Morton order, and the water volume sum(h0 * area)."""
+    import numpy as np
+    leaves = sim.get("leaves").astype(np.int64)
+    lev = (np.floor(np.log2(3 * (leaves - 3) + 1) / 2)).astype(np.int64)
+    reps = 4 ** (L - lev)
+    eta0 = sim.get("A.eta0")[leaves].astype(np.float64)
+    z0 = sim.get("z0")[leaves].astype(np.float64)
+    h0 = eta0 - z0
+    qx = sim.get("A.qx0")[leaves].astype(np.float64)
+    qy = sim.get("A.qy0")[leaves].astype(np.float64)
+    vol = float(np.sum(h0 * reps))          # dx = 1 m: a level-l cell has area 4^(L-l)
+    return np.repeat(h0, reps), np.repeat(qx, reps), np.repeat(qy, reps), vol
+
+
+def accuracy_vs_fp64(par: str, L: int, steps: int, f32) -> dict:
+    """FP32 product against the FP64 product (bit-identical to the FP64
+    oracle) after the same number of steps of the same case."""
+    import numpy as np
+    import mwdg2
+    h32, qx32, qy32, v32 = f32
+    s64 = mwdg2.Simulation(par, "f64")
+    fl = ((1 << (2 * L)) - 1) // 3 + 3           # first finest node: the initial state lives there
+    v0 = float(np.sum(s64.get("A.eta0")[fl:].astype(np.float64) - s64.get("z0")[fl:]))
+    s64.step(steps)
+    s64.sync()
+    h64, qx64, qy64, v64 = finest_fields(s64, L)
+    t64 = s64.counters()["time"]
+    s64.close()
+    return {"steps": steps, "time_fp64": t64,
+            "max_abs_dh": float(np.max(np.abs(h32 - h64))), "max_abs_dqx": float(np.max(np.abs(qx32 - qx64))),
+            "max_abs_dqy": float(np.max(np.abs(qy32 - qy64))),
+            "volume_fp32": v32, "volume_fp64": v64, "volume_rel_diff": (v32 - v64) / v64,
+            "volume_change_fp64": v64 - v0}
+
+
 def run_cpu_baseline(par: str, L: int, budget_s: float) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -115,6 +152,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-all", action="store_true", help="time every kernel (adds event overhead)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip comparing the FP32 state after the run with an FP64 run of the same steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -162,6 +201,10 @@ def main():
                 "alg_bytes_per_launch": int(kern[dom]["alg_bytes_per_launch"]),
                 "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
 
+    acc = None
+    if not args.no_accuracy:
+        acc = accuracy_vs_fp64(par, args.L, args.warmup + args.steps, finest_fields(sim, args.L))
+
     copy_gbs = mwdg2.stream_copy_gbs(1 << 30, 20)
     if roof:
         roof["measured_copy_gbs"] = round(copy_gbs, 1)
@@ -195,6 +238,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if acc is not None:
+        out["accuracy_fp32_vs_fp64"] = acc
     print(json.dumps(out))
 
 

@@ -472,18 +472,29 @@ struct Ctx : CtxBase {
     });
   }
   int top_level() const { return std::min(TOP_LEVEL, L - 1); }
+  // Levels top_level()+1 .. L-1 in bands of at most 6 levels, deepest first;
+  // a band [lo, hi] runs one work-group per level-lo node.
+  std::vector<std::pair<int, int>> bands() const {
+    std::vector<std::pair<int, int>> b;
+    for (int hi = L - 1; hi > top_level();) {
+      const int lo = std::max(top_level() + 1, hi - 5);
+      b.push_back({lo, hi});
+      hi = lo - 1;
+    }
+    return b;
+  }
   void encode_launch() {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
-    for (int level = L - 1; level > top_level(); --level) {
-      const long long n = 1LL << (2 * level);
+    for (const auto& b : bands()) {
+      const int nwg = 1 << (2 * b.first);
       timed("encode", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_encode<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             preflag, level, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
+          hipLaunchKernelGGL((k_encode_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                             xflag, preflag, b.first, b.second, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
         else
-          hipLaunchKernelGGL((k_encode<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             preflag, level, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
+          hipLaunchKernelGGL((k_encode_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                             xflag, preflag, b.first, b.second, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       });
     }
     timed("encode_top", [&] {
@@ -506,15 +517,16 @@ struct Ctx : CtxBase {
         hipLaunchKernelGGL((k_topdown_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
     });
-    for (int level = top_level() + 1; level < L; ++level) {
-      const long long n = 1LL << (2 * level);
+    const auto bs = bands();
+    for (auto it = bs.rbegin(); it != bs.rend(); ++it) {
+      const int nwg = 1 << (2 * it->first);
       timed("decode", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_topdown<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             leaflev, level, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
+          hipLaunchKernelGGL((k_topdown_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                             xflag, leaflev, it->first, it->second, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
         else
-          hipLaunchKernelGGL((k_topdown<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], prev, nw, xflag,
-                             leaflev, level, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
+          hipLaunchKernelGGL((k_topdown_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                             xflag, leaflev, it->first, it->second, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
       });
     }
   }
@@ -564,10 +576,11 @@ struct Ctx : CtxBase {
     if (!mwm) main_h = other;
   }
   void encode_scale_launch() {
-    for (int level = L - 1; level > top_level(); --level) {
-      const long long n = 1LL << (2 * level);
+    for (const auto& b : bands()) {
+      const int nwg = 1 << (2 * b.first);
       timed("encode_scale", [&] {
-        hipLaunchKernelGGL((k_encode_scale<real>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[1], sig[cur], level, cnt[2]);
+        hipLaunchKernelGGL((k_encode_scale_band<real>), dim3(nwg), dim3(WG), 0, stream, st, hier[1], sig[cur], b.first,
+                           b.second, cnt[2]);
       });
     }
     timed("encode_scale_top", [&] {
