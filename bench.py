@@ -48,7 +48,7 @@ def make_case(L: int) -> str:
     return par
 
 
-def kernel_bytes(s: int, leaves: int, wet1: int, wet2: int) -> dict:
+def kernel_bytes(s: int, L: int, leaves: int, wet1: int, wet2: int) -> dict:
     """Algorithmic HBM bytes of the update kernels over the timed steps.
 
     Every logical element a kernel must touch is counted once (neighbour
@@ -59,7 +59,11 @@ def kernel_bytes(s: int, leaves: int, wet1: int, wet2: int) -> dict:
     also reads its 9 stage-start coefficients from A.
     """
     i = 4
+    D = ((1 << (2 * L)) - 1) // 3 + 3
     return {
+        # leaf emission (labels of the level L-1 nodes read, leaf + 4 neighbour
+        # indices written) fused with the stage-1 sweep
+        "emit_sweep1": (D >> 2) + leaves * 5 * i + leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * i,
         "sweep1": leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * i,
         "wet1": wet1 * (12 * s + 9 * s),
         "sweep2": leaves * 12 * s + (leaves - wet2) * (9 * s + 9 * s) + wet2 * i,
@@ -184,7 +188,7 @@ def main():
 
     leaves = c1["leaf_updates"] - c0["leaf_updates"]
     wet = [c1["wet_updates"][q] - c0["wet_updates"][q] for q in range(2)]
-    alg = kernel_bytes(4, leaves, wet[0], wet[1])
+    alg = kernel_bytes(4, args.L, leaves, wet[0], wet[1])
     kern = {}
     for k in alg:
         if k in prof:

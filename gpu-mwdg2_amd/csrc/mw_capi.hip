@@ -78,7 +78,8 @@ struct CtxBase {
   template <typename F>
   void timed(const char* name, F&& launch) {
     const bool update = !std::strncmp(name, "stage", 5) || !std::strncmp(name, "fv1", 3) ||
-                        !std::strncmp(name, "sweep", 5) || !std::strncmp(name, "wet", 3);
+                        !std::strncmp(name, "sweep", 5) || !std::strncmp(name, "wet", 3) ||
+                        !std::strncmp(name, "emit_sweep", 10);
     if (!profiling || (profiling == 2 && !update)) { launch(); return; }
     hipEvent_t a = ev(), b = ev();
     MW_HIP(hipEventRecord(a, stream));
@@ -208,14 +209,18 @@ struct Ctx : CtxBase {
       const int G = leaf_grid(), per = G / 8;
       const long long tmax = nblk(N);
       const int cap_tiles = (int)(((tmax + 7) / 8 + per - 1) / per) + 1;
+      // regions: G sweep groups of cap entries, or one per emission tile
+      // (4 * EMIT_TILE leaves at most) when the sweep runs inside the emission
+      const int nwg = std::max(G, emit_tiles());
+      const size_t raw = std::max((size_t)G * cap_tiles * WG, (size_t)emit_tiles() * 4 * EMIT_TILE);
       for (int q = 0; q < 2; ++q) {
         wet[q].cap = cap_tiles * WG;
-        wet[q].raw = alloc<int>((size_t)G * wet[q].cap);
-        wet[q].cnt = alloc<int>(G);
-        wet[q].off = alloc<int>(G + 1);
+        wet[q].raw = alloc<int>(raw);
+        wet[q].cnt = alloc<int>(nwg);
+        wet[q].off = alloc<int>(nwg + 1);
         wet[q].list = alloc<int>(N);
         wet[q].sum = &st->wet_sum[q];
-        MW_HIP(hipMemsetAsync(wet[q].off, 0, sizeof(int) * (G + 1), stream));
+        MW_HIP(hipMemsetAsync(wet[q].off, 0, sizeof(int) * (nwg + 1), stream));
       }
     }
     for (int q = 0; q < 3; ++q) {
@@ -530,38 +535,60 @@ struct Ctx : CtxBase {
       });
     }
   }
-  void emit_launch() {
+  int emit_tiles() const { return nblk(1 << (2 * (L - 1)), EMIT_TILE); }
+  bool fuse_sweep1() const { return mwm && !hc.limitslopes; }
+  // Leaf emission; with `sweep` (MWDG2 without limiter) the stage-1 sweep of
+  // the emitted leaves runs in the same kernel.
+  void emit_launch(bool sweep = false) {
     const int Q = 1 << (2 * (L - 1));
-    const int tiles = nblk(Q, EMIT_TILE);
+    const int tiles = emit_tiles();
     int* done = &st->done;
     int* nl = &st->n_leaves;
     timed("emit_count", [&] { hipLaunchKernelGGL(k_emit_count, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums); });
     timed("emit_scan", [&] { hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles); });
+    if (sweep) {
+      WetList W = wet[0];
+      W.cap = 4 * EMIT_TILE;
+      timed("emit_sweep1", [&] {
+        hipLaunchKernelGGL((k_emit_sweep1<real>), dim3(tiles), dim3(WG), 0, stream, st, P, leaflev, Q, tile_sums, leaf,
+                           nbr[0], nbr[1], nbr[2], nbr[3], hier[main_h], hier[main_h ^ 1], zh, W);
+      });
+      return;
+    }
     timed("emit_write", [&] {
       hipLaunchKernelGGL(k_emit_write, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums, leaf, nbr[0],
                          nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
     });
   }
   int leaf_grid() const { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
-  void wet_list_launch(int q) {
-    const int G = leaf_grid();
+  // packs the wet list of stage q written by nwg work-groups with regions of cap entries
+  void wet_list_launch(int q, int nwg, int cap) {
+    WetList W = wet[q];
+    W.cap = cap;
     timed("wet_list", [&] {
-      hipLaunchKernelGGL(k_wet_scan, dim3(1), dim3(1024), 0, stream, &st->done, wet[q], G);
-      hipLaunchKernelGGL(k_wet_gather, dim3(G), dim3(WG), 0, stream, &st->done, wet[q]);
+      hipLaunchKernelGGL(k_wet_scan, dim3(1), dim3(1024), 0, stream, &st->done, W, nwg);
+      hipLaunchKernelGGL(k_wet_gather, dim3(nwg), dim3(WG), 0, stream, &st->done, W);
     });
   }
-  void stage1_launch() {
+  // swept: the stage-1 sweep already ran inside the leaf emission
+  void stage1_launch(bool swept = false) {
     const int other = main_h ^ 1;
     const int G = leaf_grid();
     if (mwm && !hc.limitslopes) {
-      timed("sweep1", [&] {
-        hipLaunchKernelGGL((k_sweep1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other],
-                           zh, wet[0]);
-      });
-      wet_list_launch(0);
+      int nwg = G;
+      if (swept) {
+        nwg = emit_tiles();
+        wet_list_launch(0, nwg, 4 * EMIT_TILE);
+      } else {
+        timed("sweep1", [&] {
+          hipLaunchKernelGGL((k_sweep1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other],
+                             zh, wet[0]);
+        });
+        wet_list_launch(0, G, wet[0].cap);
+      }
       timed("wet1", [&] {
         hipLaunchKernelGGL((k_wet1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other], zh,
-                           wet[0], G);
+                           wet[0], nwg);
       });
       return;
     }
@@ -605,25 +632,26 @@ struct Ctx : CtxBase {
       hipLaunchKernelGGL((k_sweep2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1],
                          red);
     });
-    wet_list_launch(1);
+    wet_list_launch(1, G, wet[1].cap);
     timed("wet2", [&] {
       hipLaunchKernelGGL((k_wet2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1], G,
                          red);
     });
   }
 
-  void rebuild() {
+  void rebuild(bool sweep = false) {
     encode_launch();
     decode_launch();
-    emit_launch();
+    emit_launch(sweep);
     cur ^= 1;
   }
 
   void one_step() {
     k_prologue_launch();
     k_sources_launch();
-    if (hc.epsilon > real(0.0) || !issued_first) rebuild();
-    stage1_launch();
+    const bool rb = hc.epsilon > real(0.0) || !issued_first;
+    if (rb) rebuild(fuse_sweep1());
+    stage1_launch(rb && fuse_sweep1());
     if (mwm) {
       if (hc.epsilon > real(0.0)) encode_scale_launch();
       stage2_launch();
