@@ -224,3 +224,38 @@ def random_case(d: str, L: int, seed: int, solver: str = "mwdg2", eps: float = 1
     par = os.path.join(d, "case.par")
     write_par(par, _par(solver, L, eps, t_end, manning, extra))
     return par
+
+
+def monai(d: str, fixture: str, solver: str = "mwdg2", eps: float = 0.0, L: int = 9,
+          sim_time: float = 22.5) -> str:
+    """Monai valley benchmark in the reference's form (cases/monai): rasters
+    from the nodal bathymetry as its raster.py makes them (datum 0.13535 m,
+    each cell the mean of its 4 nodes, %.15f text), the .bci/.bdy/.stage of
+    the fixture and the .par of run-functional-tests.py."""
+    os.makedirs(d, exist_ok=True)
+    nrows, ncols = 243 + 1, 392 + 1
+    z = np.load(os.path.join(fixture, "bed.npz"))["z"]
+    bed = z.reshape(ncols, nrows).T.copy()
+    bed *= -1.0
+    bed += 0.13535
+    depth = np.maximum(0.13535 - bed, 0.0)
+    bed[(bed - (-30.0)) < 1e-10] = 100.0
+
+    def project(a):
+        return 0.25 * (a[:-1, :-1] + a[:-1, 1:] + a[1:, :-1] + a[1:, 1:])
+
+    for name, a in (("monai.dem", project(bed)), ("monai.start", project(depth))):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(f"ncols        {ncols - 1}\nnrows        {nrows - 1}\nxllcorner    0\nyllcorner    0\n"
+                    f"cellsize     0.014\nNODATA_value -9999\n")
+            np.savetxt(f, a[::-1, :], fmt="%.15f")
+    for name in ("monai.bci", "monai.bdy", "monai.stage"):
+        with open(os.path.join(fixture, name)) as fi, open(os.path.join(d, name), "w") as fo:
+            fo.write(fi.read())
+    par = os.path.join(d, "monai.par")
+    write_par(par, ["monai", solver, "cuda", "cumulative", "refine_wall", ("ref_thickness", 16), ("max_ref_lvl", L),
+                    ("epsilon", repr(eps)), ("wall_height", "0.5"), ("initial_tstep", "1"), ("fpfric", "0.01"),
+                    ("sim_time", repr(sim_time)), ("massint", "0.1"), ("saveint", repr(sim_time)),
+                    ("DEMfile", "monai.dem"), ("startfile", "monai.start"), ("bcifile", "monai.bci"),
+                    ("bdyfile", "monai.bdy"), ("stagefile", "monai.stage")])
+    return par

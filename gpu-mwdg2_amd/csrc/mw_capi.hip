@@ -10,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../../include/mwdg2.h"
@@ -109,6 +110,10 @@ struct CtxBase {
   virtual void set_initial(const void* const* modal) = 0;
   virtual void set_boundaries(const mw_boundary* b) = 0;
   virtual void set_sources(int n, const mw_point_source* s) = 0;
+  virtual int stage_records(double* out, int max_rows, int* n_points) = 0;
+  virtual const std::vector<int>& gauge_codes() const = 0;
+  virtual double dx() const = 0;
+  virtual std::string out_path() const = 0;
 };
 
 template <typename real>
@@ -135,6 +140,10 @@ struct Ctx : CtxBase {
   real* xbuf = nullptr;                       // one coefficient as a plain array (parity transfers)
   real* red = nullptr;                        // [5][2 * leaf_grid()] next-step reduction slots
   WetList wet[2]{};                           // wet-leaf lists of the two MWDG2 stages
+  int n_stage = 0;                            // gauges (stage points)
+  int* stage_codes = nullptr;
+  real* rec = nullptr;                        // stage-output rows: time, depths, elevations
+  int rec_cap = 0;
   unsigned long long* cnt[3] = {nullptr, nullptr, nullptr};  // P_e, P_d, rescaled parents (slot arrays)
   // series and sources on the device
   int *s_off = nullptr, *s_len = nullptr;
@@ -228,6 +237,14 @@ struct Ctx : CtxBase {
       MW_HIP(hipMemsetAsync(cnt[q], 0, sizeof(unsigned long long) * CNT_SLOTS, stream));
     }
     if (hc.limitslopes && mwm) uprev = alloc<real>((size_t)9 * N);
+    n_stage = (int)hc.stage_codes.size();
+    if (n_stage > 0 && hc.massint_count0 == 0 && hc.massint > real(0)) {
+      const double rows = std::floor((double)hc.sim_time / (double)hc.massint) + 3.0;
+      rec_cap = (int)std::min(rows, (double)(1 << 20));
+      rec = alloc<real>((size_t)rec_cap * (1 + 2 * n_stage));
+      stage_codes = alloc<int>(n_stage);
+      MW_HIP(hipMemcpyAsync(stage_codes, hc.stage_codes.data(), sizeof(int) * n_stage, hipMemcpyHostToDevice, stream));
+    }
     fill_params();
     upload_sources_and_series();
     build_initial();
@@ -252,6 +269,8 @@ struct Ctx : CtxBase {
     P.CFL = hc.CFL; P.tol_Krivo = hc.tol_Krivo;
     P.eps_extra_factor = std::pow(real(2.0), real(1.5) + 1);          // extra_significance_kernel.cu:23
     P.limitslopes = hc.limitslopes ? 1 : 0;
+    P.monai = hc.monai ? 1 : 0;
+    P.massint = hc.massint;
     for (int k = 0; k < 4; ++k) { P.btype[k] = hc.bdy[k].type; P.bstart[k] = hc.bdy[k].start; P.bend[k] = hc.bdy[k].end; }
   }
 
@@ -457,6 +476,8 @@ struct Ctx : CtxBase {
     s.steps = 0;
     for (int k = 0; k < 4; ++k) { s.inlet[k] = hc.bdy[k].inlet; s.row[k] = 0; }
     s.n_src = n_src;
+    s.save_count = hc.massint_count0;
+    s.n_records = 0;
     MW_HIP(hipMemcpyAsync(st, &s, sizeof s, hipMemcpyHostToDevice, stream));
     MW_HIP(hipStreamSynchronize(stream));
     main_h = 0;
@@ -656,8 +677,32 @@ struct Ctx : CtxBase {
       if (hc.epsilon > real(0.0)) encode_scale_launch();
       stage2_launch();
     }
+    if (rec) {
+      timed("stage_record", [&] {
+        hipLaunchKernelGGL((k_stage_record<real>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh, leaflev,
+                           stage_codes, n_stage, rec, rec_cap);
+      });
+    }
     issued_first = true;
   }
+
+  // rows of (time, depth at each gauge, bed elevation at each gauge)
+  int stage_records(double* out, int max_rows, int* n_points) override {
+    const DevState<real> s = read_state();
+    *n_points = n_stage;
+    const int rows = rec ? s.n_records : 0;
+    if (out && rows > 0) {
+      const int k = std::min(rows, max_rows);
+      std::vector<real> buf((size_t)k * (1 + 2 * n_stage));
+      MW_HIP(hipMemcpyAsync(buf.data(), rec, sizeof(real) * buf.size(), hipMemcpyDeviceToHost, stream));
+      MW_HIP(hipStreamSynchronize(stream));
+      for (size_t i = 0; i < buf.size(); ++i) out[i] = (double)buf[i];
+    }
+    return rows;
+  }
+  double dx() const override { return (double)hc.dx; }
+  const std::vector<int>& gauge_codes() const override { return hc.stage_codes; }
+  std::string out_path() const override { return hc.dirroot + "/" + hc.resroot + ".stage"; }
 
   void step(int n) override {
     for (int i = 0; i < n; ++i) one_step();
@@ -1118,6 +1163,43 @@ int mw_filter_table(int precision, void* out144) {
 }
 
 // Drop-in for the reference binary (MAIN.cu:16-22, run_simulation.cu:3-800).
+int mw_stage_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows, int* n_points) {
+  MW_NEED(ctx && n_rows && n_points && max_rows >= 0 && (max_rows == 0 || out));
+  return guarded([&] { *n_rows = ctx->impl->stage_records(out, max_rows, n_points); });
+}
+
+int mw_write_stage_file(mw_ctx* ctx, const char* path) {
+  MW_NEED(ctx);
+  return guarded([&] {
+    int np = 0;
+    const int rows = ctx->impl->stage_records(nullptr, 0, &np);
+    if (np == 0) return;
+    std::vector<double> r((size_t)std::max(rows, 1) * (1 + 2 * np));
+    if (rows > 0) ctx->impl->stage_records(r.data(), rows, &np);
+    const std::string p = path ? std::string(path) : ctx->impl->out_path();
+    const size_t slash = p.rfind('/');
+    if (slash != std::string::npos && slash > 0) mkdir(p.substr(0, slash).c_str(), 0755);   // dirroot
+    FILE* f = std::fopen(p.c_str(), "w");
+    if (!f) throw mw::InputError("cannot write " + p);
+    // output/write_stage_point_data.cu:56-120 layout
+    std::fprintf(f, "Stage output, depth (m).\n\nStage information (stage,x,y,elev):\n");
+    const std::vector<int>& codes = ctx->impl->gauge_codes();
+    const double dx = ctx->impl->dx();
+    for (int k = 0; k < np; ++k) {
+      const int i = mw::compact(codes[k]), j = mw::compact(codes[k] >> 1);
+      std::fprintf(f, "%d %f %f %f\n", k + 1, i * dx + dx / 2.0, j * dx + dx / 2.0, rows > 0 ? r[1 + np + k] : 0.0);
+    }
+    std::fprintf(f, "\nOutput, depths:\nTime; stages 1 to %d\n", np);
+    for (int row = 0; row < rows; ++row) {
+      const double* x = r.data() + (size_t)row * (1 + 2 * np);
+      std::fprintf(f, "%f ", x[0]);
+      for (int k = 0; k < np; ++k) std::fprintf(f, (k + 1 == np) ? "%f" : "%f ", x[1 + k]);
+      std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+  });
+}
+
 int mw_run_simulation(int argc, const char* const* argv) {
   MW_NEED(argc >= 2 && argv);
   const char* par = argv[argc - 1];
@@ -1146,6 +1228,7 @@ int mw_run_simulation(int argc, const char* const* argv) {
     std::printf("Elements: %lld, compression: %f%%, time step: %f, timestep: %lld, sim time: %f\n", c.n_leaves,
                 100.0 - 100.0 * c.n_leaves / (n * n), c.dt_next, c.steps + 1, c.time);
     std::printf("Loop time: %f s\n", secs);
+    if (mw_write_stage_file(ctx, nullptr) != MW_OK) std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
   } else {
     std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
   }

@@ -208,7 +208,13 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
   c.manning = par_real<real>(par, "fpfric");
   c.monai = par_has(par, "monai");
   c.oregon = par_has(par, "oregon-seaside");
-  if (c.monai || c.oregon) throw InputError("the Monai and Oregon wavemaker boundaries are not supported");
+  if (c.oregon) throw InputError("the Oregon wavemaker boundary is not supported");
+  if (par_has(par, "massint")) {
+    c.massint = par_real<real>(par, "massint");
+    c.massint_count0 = 0;
+  }
+  if (par_has(par, "dirroot")) c.dirroot = par_str(par, "dirroot");
+  if (par_has(par, "resroot")) c.resroot = par_str(par, "resroot");
 
   for (int i = 0; i + 1 < argc; ++i) {                       // input/read_command_line_params.cpp:12-47
     if (!std::strncmp(argv[i], "-epsilon", 8)) {

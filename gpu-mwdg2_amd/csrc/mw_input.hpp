@@ -48,6 +48,11 @@ struct HostCase {
   real xmin = 0, ymin = 0, sim_time = 0, manning = 0, dx = 0, g = real(9.80665);
   int xsz = 0, ysz = 0;
   bool monai = false, oregon = false;
+  // stage output every massint seconds (classes/SaveInterval.h); interval
+  // 9999 and count 9999 when the .par names no massint, as the reference
+  real massint = real(9999.0);
+  int massint_count0 = 9999;
+  std::string dirroot = "res", resroot = "res";   // classes/PlottingParams.h:17-45
   HostBoundary<real> bdy[4];          // N, E, S, W
   std::vector<HostSource<real>> srcs;
   std::vector<int> stage_codes;

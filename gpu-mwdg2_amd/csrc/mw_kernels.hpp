@@ -26,6 +26,8 @@ struct DevState {
   int n_src;
   unsigned long long leaf_sum;    // leaves updated, summed over steps
   unsigned long long wet_sum[2];  // leaves of the wet lists of stage 1 and 2, summed over steps
+  int save_count;                 // stage-output counter (classes/SaveInterval.h:47-56)
+  int n_records;                  // stage-output rows written
 };
 
 // Time series of the boundaries (entries 0..3) and point sources (4..).
@@ -46,6 +48,8 @@ struct Params {
   int limitslopes;
   int btype[4];                   // N, E, S, W boundary type
   int bstart[4], bend[4];
+  int monai;                      // west HFIX/HVAR is the Monai non-reflective wavemaker
+  real massint;                   // stage-output interval
 };
 
 // Flow scale coefficients of every node of the hierarchy, eta form. Coefficient

@@ -333,6 +333,52 @@ MW_HD void apply_friction(real h, real tol_h, real tol_s, real& qx, real& qy, re
 // Flow coefficients of one cell, h form: c[v][m], v = h,qx,qy; m = 0,1x,1y.
 template <typename real> struct Cell { real c[3][3]; real z[3]; };
 
+// Monai valley wavemaker (operators/non_reflective_wave.cuh:15-93): a Newton
+// iteration (at most 101 steps, tolerance 1e-6 on the step and on the
+// residual) locates the foot p of the outgoing characteristic inside the
+// boundary cell, linearly between the still-water state (hb = 0.13535 m, at
+// rest) outside and the cell state (hp, up); the ghost state then follows
+// from the outgoing invariant u - 2c there and the incoming one built from
+// the inlet surface elevation. pow(x, -0.5) is evaluated as 1 / sqrt(x).
+template <typename real>
+MW_HD void monai_wave(real h_inlet, real dt, real dxl, real hp, real up, real g, real& h_out, real& u_out) {
+  const real hb = real(0.13535), ub = real(0.0);
+  const real xa = -dxl / real(2.0), tol = real(1e-6);
+  real p0 = dxl / real(2.0), p1 = p0 - dxl / real(2.0);
+  real r = (p0 - xa) / dxl;
+  real f0 = (p0 - xa) + dt * ((ub - r * (ub - up)) - rsqrt_cr<real>(g * (hb - r * (hb - hp))));
+  int cond = 0;
+  for (int k = 0; k <= 100 && cond == 0; ++k) {
+    r = (p0 - xa) / dxl;
+    const real df = real(1.0) + dt * ((up - ub) / dxl - real(0.5) * (real(1.0) / rsqrt_cr<real>(g * (hb - r * (hb - hp)))) *
+                                                             (g * (hp - hb) / dxl));
+    real dp;
+    if (rabs<real>(df) <= real(0.0)) {
+      cond = 1;
+      dp = p1 - p0;
+      p1 = p0;
+    } else {
+      dp = f0 / df;
+      p1 = p0 - dp;
+    }
+    const real r1 = (p1 - xa) / dxl;
+    const real f1 = (p1 - xa) + dt * ((ub - r1 * (ub - up)) - rsqrt_cr<real>(g * (hb - r1 * (hb - hp))));
+    const real relerr = rabs<real>(dp) / (rabs<real>(p1) + real(0.0));
+    if (relerr < tol) cond = 2;
+    if (rabs<real>(f1) < tol) cond = 3;
+    if (relerr <= tol && rabs<real>(f1) < tol) cond = 4;
+    p0 = p1;
+    f0 = f1;
+  }
+  const real hr = hb - (hb - hp) * (p1 - xa) / dxl;
+  const real ur = ub - (ub - up) * (p1 - xa) / dxl;
+  const real cr = rsqrt_cr<real>(g * hr);
+  const real bm = ur - real(2.0) * cr;
+  const real bp = real(2.0) * rsqrt_cr<real>(g * (h_inlet + real(0.13535))) + h_inlet * rsqrt_cr<real>(g / real(0.13535));
+  h_out = real(1.0) / real(16.0) / g * (bp - bm) * (bp - bm);
+  u_out = real(0.5) * (bp + bm);
+}
+
 // MWDG2 implicit Manning on one cell's mean and its 4 Gauss-like points
 // (operators/friction_implicit.cu:191-315).
 template <typename real>

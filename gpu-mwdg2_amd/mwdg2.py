@@ -82,6 +82,8 @@ def lib():
         L.mw_filter_table.argtypes = [i, vp]
         L.mw_stream_copy.argtypes = [ll, i, ctypes.POINTER(d)]
         L.mw_run_simulation.argtypes = [i, ctypes.POINTER(cp)]
+        L.mw_stage_records.argtypes = [vp, ctypes.POINTER(d), i, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.mw_write_stage_file.argtypes = [vp, cp]
         _lib = L
     return _lib
 
@@ -168,6 +170,19 @@ class Simulation:
         arrs = [np.ascontiguousarray(m, dtype=self.real) for m in modal]
         ptrs = (ctypes.c_void_p * 12)(*[a.ctypes.data for a in arrs])
         _check(lib().mw_set_initial(self._h, ptrs))
+
+    def stage_records(self) -> np.ndarray:
+        """Rows of (time, depth at each gauge, bed elevation at each gauge)."""
+        rows, npt = ctypes.c_int(), ctypes.c_int()
+        _check(lib().mw_stage_records(self._h, None, 0, ctypes.byref(rows), ctypes.byref(npt)))
+        out = np.zeros((max(rows.value, 1), 1 + 2 * npt.value))
+        if rows.value:
+            _check(lib().mw_stage_records(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), rows.value,
+                                          ctypes.byref(rows), ctypes.byref(npt)))
+        return out[:rows.value]
+
+    def write_stage_file(self, path: str | None = None):
+        _check(lib().mw_write_stage_file(self._h, path.encode() if path else None))
 
     # profiling -----------------------------------------------------------
     def profile(self, level: int = 1):

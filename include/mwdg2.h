@@ -152,6 +152,17 @@ int mw_profile_read(mw_ctx* ctx, mw_kernel_time* out, int max_entries, int* n_en
  * (mra/Filters.h:202-397); 144 doubles (precision 8) or floats (precision 4). */
 int mw_filter_table(int precision, void* out144);
 
+/* Stage (gauge) output recorded on the device at the end of every step whose
+ * time reaches massint * count (run_simulation.cu:707-750,
+ * classes/SaveInterval.h:47-56): rows of 1 + 2 * n_points doubles: time, the
+ * depth h0 of the leaf holding each gauge's finest cell, then that leaf's z0
+ * (output/project_assem_sol.cu:60-77). *n_rows receives the number of rows
+ * recorded; at most max_rows are copied. */
+int mw_stage_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows, int* n_points);
+/* Writes the recorded rows as the reference's stage file
+ * (output/write_stage_point_data.cu:37-190); path NULL = dirroot/resroot.stage. */
+int mw_write_stage_file(mw_ctx* ctx, const char* path);
+
 /* Drop-in for the reference executable: argv[argc-1] is the .par file
  * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */
 int mw_run_simulation(int argc, const char* const* argv);

@@ -31,6 +31,7 @@ std::vector<real>* real_array(Oracle<real>& o, const std::string& n) {
   for (int i = 0; i < 12; ++i)
     if (n == hn[i]) return hier[i];
   if (n == "norm") return &o.norm;
+  if (n == "stage") return &o.stage_rec;
   if (n == "dt_cfl") return &o.dt_cfl;
   // det_<var>_<mode>_<family>, var in eta,qx,qy,z; mode 0,1x,1y; family a,b,g
   if (n.rfind("det_", 0) == 0) {

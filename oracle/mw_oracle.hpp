@@ -132,6 +132,8 @@ struct Params {
   int  xsz = 0, ysz = 0;
   real g = real(9.80665), time = 0, manning = 0;
   bool is_monai = false, is_oregon = false;
+  real massint = real(9999.0);                // classes/SaveInterval.h: 9999/9999 without massint
+  int massint_count0 = 9999;
   real dx = 0;                            // input/read_cell_size.cpp:3-45
 };
 
@@ -202,6 +204,9 @@ struct Oracle {
   Bdy<real> bN, bE, bS, bW;
   std::vector<PointSrc<real>> srcs;
   std::vector<int> stage_codes;
+  int save_count = 9999;
+  std::vector<real> stage_rec;                // rows: time, depths, elevations
+  void record_stage();
 
   int N = 0, D = 0, H = 0, mesh_dim = 0;
   // hierarchy (classes/ScaleCoefficients.h:47-60)

@@ -106,3 +106,30 @@ def test_run_until_end_time(case_dir):
     assert c["done"] == 1
     assert c["time"] == t_o and abs(t_o - 0.05) < 1e-12
     assert c["steps"] == n
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_monai_wavemaker_and_stage_rows(case_dir, solver):
+    """West HVAR boundary as the Monai non-reflective wavemaker
+    (add_ghost_cells.cu:450-469) and the gauge rows of the stage output
+    (run_simulation.cu:707-750), against the oracle bit for bit."""
+    import mwdg2
+    import oracle
+    bci = "W 0 64 HVAR wave\n"
+    bdy = "wave\n4 seconds\n0.0 0\n0.01 0.2\n-0.005 0.5\n0.0 2.0\n"
+    par = _mk(case_dir, f"monaiw_{solver}", cases.random_case, L=6, seed=23, solver=solver, manning=0.01,
+              t_end=2.0, bci=bci, bdy=bdy, stage=3)
+    txt = open(par).read().replace("massint       2.0", "massint       0.05")
+    with open(par, "w") as f:
+        f.write(txt + "monai\n")
+    parity_util.run_pair(par, "f64", steps=25)
+    o = oracle.Oracle(par, "f64")
+    with mwdg2.Simulation(par, "f64") as s:
+        s.run(max_steps=100000, batch=16)
+        rows = s.stage_records()
+    while o.step():
+        pass
+    ro = o.get("stage").reshape(-1, rows.shape[1])
+    o.close()
+    assert rows.shape[0] >= 8
+    assert np.array_equal(rows, ro)
