@@ -487,7 +487,7 @@ struct Ctx : CtxBase {
   // ---- step pieces ---------------------------------------------------------
   void k_prologue_launch() {
     timed("prologue", [&] {
-      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(WG), 0, stream, st, P, series(), sources(), red,
+      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(PRO_WG), 0, stream, st, P, series(), sources(), red,
                          2 * leaf_grid());
     });
   }
