@@ -112,6 +112,8 @@ struct CtxBase {
   virtual void set_sources(int n, const mw_point_source* s) = 0;
   virtual int stage_records(double* out, int max_rows, int* n_points) = 0;
   virtual const std::vector<int>& gauge_codes() const = 0;
+  virtual void write_rasters(int index) = 0;
+  virtual bool raster_due() = 0;
   virtual double dx() const = 0;
   virtual std::string out_path() const = 0;
 };
@@ -702,6 +704,56 @@ struct Ctx : CtxBase {
   }
   double dx() const override { return (double)hc.dx; }
   const std::vector<int>& gauge_codes() const override { return hc.stage_codes; }
+
+  // output/write_all_raster_maps_{dg2,fv1}.cu + write_raster_file.cu:
+  // dirroot/resroot-<index>.<elev|wd|qx|qy|vx|vy>
+  real* rbuf = nullptr;
+  int raster_count = -1;
+  void write_rasters(int index) override {
+    const int n = hc.xsz * hc.ysz;
+    if (!rbuf) rbuf = alloc<real>((size_t)6 * n);
+    real* o[6];
+    for (int k = 0; k < 6; ++k) o[k] = rbuf + (size_t)k * n;
+    if (mwm)
+      hipLaunchKernelGGL((k_raster_project<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh, leaflev,
+                         o[0], o[1], o[2], o[3], o[4], o[5]);
+    else
+      hipLaunchKernelGGL((k_raster_project<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh,
+                         leaflev, o[0], o[1], o[2], o[3], o[4], o[5]);
+    MW_HIP(hipGetLastError());
+    std::vector<real> h((size_t)6 * n);
+    MW_HIP(hipMemcpyAsync(h.data(), rbuf, sizeof(real) * h.size(), hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    const char* ext[6] = {"elev", "wd", "qx", "qy", "vx", "vy"};
+    const bool on[6] = {!hc.elevoff, !hc.depthoff, hc.qoutput, hc.qoutput, hc.voutput, hc.voutput};
+    mkdir(hc.dirroot.c_str(), 0755);
+    for (int k = 0; k < 6; ++k) {
+      if (!on[k]) continue;
+      const std::string path = hc.dirroot + "/" + hc.resroot + "-" + std::to_string(index) + "." + ext[k];
+      FILE* f = std::fopen(path.c_str(), "w");
+      if (!f) throw InputError("cannot write " + path);
+      std::fprintf(f, "ncols        %d\nnrows        %d\n", hc.xsz, hc.ysz);
+      std::fprintf(f, "xllcorner    %f\nyllcorner    %f\ncellsize     %f\nNODATA_value %d\n", (double)hc.xmin,
+                   (double)hc.ymin, (double)hc.dx, -9999);
+      const char* fmt = sizeof(real) == 4 ? "%.8f " : "%.15f ";
+      for (int j = 0; j < hc.ysz; ++j) {
+        const real* row = h.data() + (size_t)k * n + (size_t)(hc.ysz - 1 - j) * hc.xsz;
+        for (int i = 0; i < hc.xsz; ++i) std::fprintf(f, fmt, (double)row[i]);
+        std::fprintf(f, "\n");
+      }
+      std::fclose(f);
+    }
+  }
+  // classes/SaveInterval.h:47-56 with the saveint interval, checked after a step
+  bool raster_due() override {
+    if (raster_count < 0) raster_count = hc.saveint_count0;
+    const DevState<real> s = read_state();
+    if (s.time >= hc.saveint * real(raster_count)) {
+      ++raster_count;
+      return true;
+    }
+    return false;
+  }
   std::string out_path() const override { return hc.dirroot + "/" + hc.resroot + ".stage"; }
 
   void step(int n) override {
@@ -1200,6 +1252,11 @@ int mw_write_stage_file(mw_ctx* ctx, const char* path) {
   });
 }
 
+int mw_write_rasters(mw_ctx* ctx, int index) {
+  MW_NEED(ctx && index >= 0);
+  return guarded([&] { ctx->impl->write_rasters(index); });
+}
+
 int mw_run_simulation(int argc, const char* const* argv) {
   MW_NEED(argc >= 2 && argv);
   const char* par = argv[argc - 1];
@@ -1213,8 +1270,19 @@ int mw_run_simulation(int argc, const char* const* argv) {
   mw_counters_get(ctx, &c);
   double n = (double)(1 << c.max_ref_lvl);
   long long last_print = -1;
+  const bool rasters = mw::par_has(par, "raster_out");
+  int saved = 0;
   while (true) {
-    rc = mw_run_until(ctx, c.steps + 10000, 64, &c);
+    if (rasters) {          // the save check needs the time after every step (run_simulation.cu:656-690)
+      rc = mw_run_until(ctx, c.steps + 1, 1, &c);
+      if (rc == MW_OK && !c.done) {
+        bool due = false;
+        rc = guarded([&] { due = ctx->impl->raster_due(); });
+        if (rc == MW_OK && due) rc = mw_write_rasters(ctx, saved++);
+      }
+    } else {
+      rc = mw_run_until(ctx, c.steps + 10000, 64, &c);
+    }
     if (rc != MW_OK) break;
     if (c.steps / 10000 != last_print) {            // run_simulation.cu:753-762
       last_print = c.steps / 10000;

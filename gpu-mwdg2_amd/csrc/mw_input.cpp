@@ -213,6 +213,15 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
     c.massint = par_real<real>(par, "massint");
     c.massint_count0 = 0;
   }
+  if (par_has(par, "saveint")) {
+    c.saveint = par_real<real>(par, "saveint");
+    c.saveint_count0 = 0;
+  }
+  c.raster_out = par_has(par, "raster_out");
+  c.voutput = par_has(par, "voutput");
+  c.qoutput = par_has(par, "qoutput");
+  c.elevoff = par_has(par, "elevoff");
+  c.depthoff = par_has(par, "depthoff");
   if (par_has(par, "dirroot")) c.dirroot = par_str(par, "dirroot");
   if (par_has(par, "resroot")) c.resroot = par_str(par, "resroot");
 

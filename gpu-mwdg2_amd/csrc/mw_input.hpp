@@ -53,6 +53,10 @@ struct HostCase {
   real massint = real(9999.0);
   int massint_count0 = 9999;
   std::string dirroot = "res", resroot = "res";   // classes/PlottingParams.h:17-45
+  // raster maps every saveint seconds (classes/PlottingParams.h:25-33)
+  bool raster_out = false, voutput = false, qoutput = false, elevoff = false, depthoff = false;
+  real saveint = real(9999.0);
+  int saveint_count0 = 9999;
   HostBoundary<real> bdy[4];          // N, E, S, W
   std::vector<HostSource<real>> srcs;
   std::vector<int> stage_codes;

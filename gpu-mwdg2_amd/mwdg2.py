@@ -84,6 +84,7 @@ def lib():
         L.mw_run_simulation.argtypes = [i, ctypes.POINTER(cp)]
         L.mw_stage_records.argtypes = [vp, ctypes.POINTER(d), i, ctypes.POINTER(i), ctypes.POINTER(i)]
         L.mw_write_stage_file.argtypes = [vp, cp]
+        L.mw_write_rasters.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -180,6 +181,9 @@ class Simulation:
             _check(lib().mw_stage_records(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), rows.value,
                                           ctypes.byref(rows), ctypes.byref(npt)))
         return out[:rows.value]
+
+    def write_rasters(self, index: int = 0):
+        _check(lib().mw_write_rasters(self._h, int(index)))
 
     def write_stage_file(self, path: str | None = None):
         _check(lib().mw_write_stage_file(self._h, path.encode() if path else None))

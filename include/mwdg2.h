@@ -163,6 +163,12 @@ int mw_stage_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows, int* n
  * (output/write_stage_point_data.cu:37-190); path NULL = dirroot/resroot.stage. */
 int mw_write_stage_file(mw_ctx* ctx, const char* path);
 
+/* Raster maps of the current leaf solution on the DEM's cells, written as
+ * dirroot/resroot-<index>.{elev,wd[,qx,qy][,vx,vy]} ESRI grids with the
+ * .par's raster keywords (output/write_all_raster_maps_{dg2,fv1}.cu,
+ * write_raster_file.cu:3-57); h is the stored eta minus z. */
+int mw_write_rasters(mw_ctx* ctx, int index);
+
 /* Drop-in for the reference executable: argv[argc-1] is the .par file
  * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */
 int mw_run_simulation(int argc, const char* const* argv);

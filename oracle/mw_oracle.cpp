@@ -256,6 +256,9 @@ int mwo_set_scalar(void* hv, const char* name, double v) {
 }
 
 // --- stateless element routines, for the known-answer tests ---------------
+void mwo_wavemaker_d(double h_in, double dt, double dxl, double hp, double up, double g, double* out2) {
+  wavemaker<double>(h_in, dt, dxl, hp, up, g, out2[0], out2[1]);
+}
 int mwo_morton(int x, int y) { return morton(x, y); }
 int mwo_compact(int code) { return compact(code); }
 int mwo_lvl_idx(int level) { return lvl_idx(level); }

@@ -57,6 +57,7 @@ def lib():
         L.mwo_flux_hll_x_d.argtypes = [vp, vp, d, d, vp]
         L.mwo_flux_hll_y_d.argtypes = [vp, vp, d, d, vp]
         L.mwo_friction_d.argtypes = [d, d, d, vp, d, d, d]
+        L.mwo_wavemaker_d.argtypes = [d, d, d, d, d, d, vp]
         _lib = L
     return _lib
 
@@ -183,6 +184,12 @@ def friction(h, qx, qy, n, dt, tol_h=1e-3, tol_s=1e-9, g=9.80665):
     q = np.array([qx, qy], dtype=np.float64)
     lib().mwo_friction_d(h, tol_h, tol_s, q.ctypes.data, n, g, dt)
     return q
+
+
+def wavemaker(h_in, dt, dxl, hp, up, g=9.80665):
+    out = np.zeros(2)
+    lib().mwo_wavemaker_d(h_in, dt, dxl, hp, up, g, out.ctypes.data)
+    return out
 
 
 def cbrt(x, dtype=np.float64):

@@ -1,0 +1,99 @@
+"""Raster maps of the leaf solution (output/write_all_raster_maps_{dg2,fv1}.cu,
+write_raster_file.cu:3-57) written by the C ABI, against the oracle's leaf
+solution evaluated on the DEM's cells with numpy: MWDG2 takes each leaf's
+planar solution at the finest cell centre, HWFV1 the leaf mean; rows are
+written from the north edge down. h is the stored eta minus z (the checker
+does the same), so the values agree to the printed digits."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def read_grid(path):
+    with open(path) as f:
+        head = [next(f).split() for _ in range(6)]
+        data = np.loadtxt(f)
+    return {k: v for k, v in head}, data
+
+
+def expected(o, L, xsz, ysz, mw):
+    n = int(o.scalar("length"))
+    act = o.get("assem.act")[:n].astype(np.int64)
+    lev = o.get("assem.lev")[:n].astype(np.int64)
+    get = lambda name: o.get("assem." + name)[:n].astype(np.float32)
+    f32 = np.float32
+    out = {k: np.zeros((ysz, xsz), dtype=np.float32) for k in ("elev", "wd", "qx", "qy", "vx", "vy")}
+    s3 = np.sqrt(f32(3.0))
+    mods = ("0", "1x", "1y") if mw else ("0",)
+    vals = {v: [get(v + m) for m in mods] for v in ("h", "qx", "qy", "z")}
+    # the hierarchy keeps eta = h + z, so h comes back as (h + z) - z
+    vals["h"] = [(hm + zm) - zm for hm, zm in zip(vals["h"], vals["z"])]
+    for e in range(n):
+        l = lev[e]
+        code = int(act[e] - (((1 << (2 * l)) - 1) // 3 + 3))
+        side = 1 << (L - l)
+        x0 = oracle.compact(code) * side
+        y0 = oracle.compact(code >> 1) * side
+        du = f32(1.0) / f32(side)
+        for jl in range(side):
+            j = y0 + jl
+            if j >= ysz:
+                continue
+            for il in range(side):
+                i = x0 + il
+                if i >= xsz:
+                    continue
+                if mw:
+                    xu = f32(il) * du + du / f32(2.0)
+                    yu = f32(jl) * du + du / f32(2.0)
+                    b = (f32(1.0), s3 * (f32(2.0) * xu - f32(1.0)), s3 * (f32(2.0) * yu - f32(1.0)))
+                    ev = lambda c: c[0][e] * b[0] + c[1][e] * b[1] + c[2][e] * b[2]
+                else:
+                    ev = lambda c: c[0][e]
+                h, z, qx, qy = ev(vals["h"]), ev(vals["z"]), ev(vals["qx"]), ev(vals["qy"])
+                out["elev"][j, i] = h + z
+                out["wd"][j, i] = h
+                out["qx"][j, i] = qx
+                out["qy"][j, i] = qy
+                out["vx"][j, i] = qx / h if h > 1e-3 else 0.0
+                out["vy"][j, i] = qy / h if h > 1e-3 else 0.0
+    return out
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_raster_maps_match_leaf_solution(case_dir, solver):
+    import mwdg2
+    d = os.path.join(str(case_dir), f"rast_{solver}")
+    par = cases.random_case(d, L=5, seed=31, solver=solver, xsz=29, ysz=23, manning=0.02, t_end=1.0)
+    with open(par, "a") as f:
+        f.write("raster_out\nqoutput\nvoutput\ndirroot       out\nresroot       r\n")
+    o = oracle.Oracle(par, "f32")
+    for _ in range(4):
+        o.step()
+    L, xsz, ysz = int(o.scalar("L")), int(o.scalar("xsz")), int(o.scalar("ysz"))
+    exp = expected(o, L, xsz, ysz, solver == "mwdg2")
+    o.close()
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        with mwdg2.Simulation(par, "f32") as s:
+            s.step(4)
+            s.write_rasters(3)
+    finally:
+        os.chdir(cwd)
+    files = sorted(glob.glob(os.path.join(d, "out", "r-3.*")))
+    assert [os.path.basename(f) for f in files] == ["r-3.elev", "r-3.qx", "r-3.qy", "r-3.vx", "r-3.vy", "r-3.wd"]
+    for f in files:
+        head, data = read_grid(f)
+        assert head["ncols"] == str(xsz) and head["nrows"] == str(ysz) and head["NODATA_value"] == "-9999"
+        ext = f.rsplit(".", 1)[1]
+        got = data[::-1, :]                     # first row written is the northern one
+        assert got.shape == (ysz, xsz)
+        assert np.allclose(got, exp[ext], rtol=2e-6, atol=2e-7), (ext, np.max(np.abs(got - exp[ext])))

@@ -197,3 +197,18 @@ def test_cfl_min_is_exact_minimum(tmp_path):
         assert o.scalar("dt") == pytest.approx(float(cand.min()), rel=1e-14)
     finally:
         o.close()
+
+
+def test_wavemaker_keeps_still_water_still():
+    """Monai wavemaker (operators/non_reflective_wave.cuh:15-93): with no
+    inlet elevation and the tank at rest at its still depth (0.13535 m), both
+    Riemann invariants are those of the rest state, so the ghost is the rest
+    state; a raised inlet raises the ghost depth and pushes water in."""
+    hb = 0.13535
+    h, u = oracle.wavemaker(0.0, 0.01, 0.014, hb, 0.0)
+    assert abs(h - hb) < 1e-12 and abs(u) < 1e-12
+    h1, u1 = oracle.wavemaker(0.005, 0.01, 0.014, hb, 0.0)
+    assert h1 > hb and u1 > 0.0
+    # a cell below the still depth draws water in through the boundary
+    h2, u2 = oracle.wavemaker(0.0, 0.01, 0.014, 0.12, 0.0)
+    assert 0.12 < h2 < hb and u2 > 0.0
