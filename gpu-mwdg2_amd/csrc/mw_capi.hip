@@ -116,6 +116,7 @@ struct CtxBase {
   virtual bool raster_due() = 0;
   virtual double dx() const = 0;
   virtual std::string out_path() const = 0;
+  virtual bool voutput_stage() const = 0;
 };
 
 template <typename real>
@@ -243,7 +244,7 @@ struct Ctx : CtxBase {
     if (n_stage > 0 && hc.massint_count0 == 0 && hc.massint > real(0)) {
       const double rows = std::floor((double)hc.sim_time / (double)hc.massint) + 3.0;
       rec_cap = (int)std::min(rows, (double)(1 << 20));
-      rec = alloc<real>((size_t)rec_cap * (1 + 2 * n_stage));
+      rec = alloc<real>((size_t)rec_cap * (1 + 4 * n_stage));
       stage_codes = alloc<int>(n_stage);
       MW_HIP(hipMemcpyAsync(stage_codes, hc.stage_codes.data(), sizeof(int) * n_stage, hipMemcpyHostToDevice, stream));
     }
@@ -272,6 +273,7 @@ struct Ctx : CtxBase {
     P.eps_extra_factor = std::pow(real(2.0), real(1.5) + 1);          // extra_significance_kernel.cu:23
     P.limitslopes = hc.limitslopes ? 1 : 0;
     P.monai = hc.monai ? 1 : 0;
+    P.oregon = hc.oregon ? 1 : 0;
     P.massint = hc.massint;
     for (int k = 0; k < 4; ++k) { P.btype[k] = hc.bdy[k].type; P.bstart[k] = hc.bdy[k].start; P.bend[k] = hc.bdy[k].end; }
   }
@@ -681,21 +683,25 @@ struct Ctx : CtxBase {
     }
     if (rec) {
       timed("stage_record", [&] {
-        hipLaunchKernelGGL((k_stage_record<real>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh, leaflev,
-                           stage_codes, n_stage, rec, rec_cap);
+        if (mwm)
+          hipLaunchKernelGGL((k_stage_record<real, true>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh,
+                             leaflev, stage_codes, n_stage, rec, rec_cap);
+        else
+          hipLaunchKernelGGL((k_stage_record<real, false>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh,
+                             leaflev, stage_codes, n_stage, rec, rec_cap);
       });
     }
     issued_first = true;
   }
 
-  // rows of (time, depth at each gauge, bed elevation at each gauge)
+  // rows of (time, depth, bed elevation, x velocity, y velocity at each gauge)
   int stage_records(double* out, int max_rows, int* n_points) override {
     const DevState<real> s = read_state();
     *n_points = n_stage;
     const int rows = rec ? s.n_records : 0;
     if (out && rows > 0) {
       const int k = std::min(rows, max_rows);
-      std::vector<real> buf((size_t)k * (1 + 2 * n_stage));
+      std::vector<real> buf((size_t)k * (1 + 4 * n_stage));
       MW_HIP(hipMemcpyAsync(buf.data(), rec, sizeof(real) * buf.size(), hipMemcpyDeviceToHost, stream));
       MW_HIP(hipStreamSynchronize(stream));
       for (size_t i = 0; i < buf.size(); ++i) out[i] = (double)buf[i];
@@ -755,6 +761,7 @@ struct Ctx : CtxBase {
     return false;
   }
   std::string out_path() const override { return hc.dirroot + "/" + hc.resroot + ".stage"; }
+  bool voutput_stage() const override { return hc.voutput_stage; }
 
   void step(int n) override {
     for (int i = 0; i < n; ++i) one_step();
@@ -1226,29 +1233,40 @@ int mw_write_stage_file(mw_ctx* ctx, const char* path) {
     int np = 0;
     const int rows = ctx->impl->stage_records(nullptr, 0, &np);
     if (np == 0) return;
-    std::vector<double> r((size_t)std::max(rows, 1) * (1 + 2 * np));
+    const size_t w = 1 + 4 * (size_t)np;
+    std::vector<double> r((size_t)std::max(rows, 1) * w);
     if (rows > 0) ctx->impl->stage_records(r.data(), rows, &np);
     const std::string p = path ? std::string(path) : ctx->impl->out_path();
     const size_t slash = p.rfind('/');
     if (slash != std::string::npos && slash > 0) mkdir(p.substr(0, slash).c_str(), 0755);   // dirroot
-    FILE* f = std::fopen(p.c_str(), "w");
-    if (!f) throw mw::InputError("cannot write " + p);
-    // output/write_stage_point_data.cu:56-120 layout
-    std::fprintf(f, "Stage output, depth (m).\n\nStage information (stage,x,y,elev):\n");
     const std::vector<int>& codes = ctx->impl->gauge_codes();
     const double dx = ctx->impl->dx();
-    for (int k = 0; k < np; ++k) {
-      const int i = mw::compact(codes[k]), j = mw::compact(codes[k] >> 1);
-      std::fprintf(f, "%d %f %f %f\n", k + 1, i * dx + dx / 2.0, j * dx + dx / 2.0, rows > 0 ? r[1 + np + k] : 0.0);
+    // output/write_stage_point_data.cu:37-190: the depth file, and with
+    // voutput_stage the x and y velocity files beside it
+    struct Out { std::string path, title, label; int col; };
+    std::vector<Out> outs{{p, "depth (m)", "depths", 0}};
+    if (ctx->impl->voutput_stage()) {
+      const std::string base = (p.size() > 6 && p.compare(p.size() - 6, 6, ".stage") == 0) ? p.substr(0, p.size() - 6) : p;
+      outs.push_back({base + ".xvelocity", "x velocity (m/s)", "x velocities", 2});
+      outs.push_back({base + ".yvelocity", "y velocity (m/s)", "y velocities", 3});
     }
-    std::fprintf(f, "\nOutput, depths:\nTime; stages 1 to %d\n", np);
-    for (int row = 0; row < rows; ++row) {
-      const double* x = r.data() + (size_t)row * (1 + 2 * np);
-      std::fprintf(f, "%f ", x[0]);
-      for (int k = 0; k < np; ++k) std::fprintf(f, (k + 1 == np) ? "%f" : "%f ", x[1 + k]);
-      std::fprintf(f, "\n");
+    for (const Out& o : outs) {
+      FILE* f = std::fopen(o.path.c_str(), "w");
+      if (!f) throw mw::InputError("cannot write " + o.path);
+      std::fprintf(f, "Stage output, %s.\n\nStage information (stage,x,y,elev):\n", o.title.c_str());
+      for (int k = 0; k < np; ++k) {
+        const int i = mw::compact(codes[k]), j = mw::compact(codes[k] >> 1);
+        std::fprintf(f, "%d %f %f %f\n", k + 1, i * dx + dx / 2.0, j * dx + dx / 2.0, rows > 0 ? r[1 + np + k] : 0.0);
+      }
+      std::fprintf(f, "\nOutput, %s:\nTime; stages 1 to %d\n", o.label.c_str(), np);
+      for (int row = 0; row < rows; ++row) {
+        const double* x = r.data() + (size_t)row * w;
+        std::fprintf(f, "%f ", x[0]);
+        for (int k = 0; k < np; ++k) std::fprintf(f, (k + 1 == np) ? "%f" : "%f ", x[1 + o.col * np + k]);
+        std::fprintf(f, "\n");
+      }
+      std::fclose(f);
     }
-    std::fclose(f);
   });
 }
 

@@ -208,7 +208,6 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
   c.manning = par_real<real>(par, "fpfric");
   c.monai = par_has(par, "monai");
   c.oregon = par_has(par, "oregon-seaside");
-  if (c.oregon) throw InputError("the Oregon wavemaker boundary is not supported");
   if (par_has(par, "massint")) {
     c.massint = par_real<real>(par, "massint");
     c.massint_count0 = 0;
@@ -219,6 +218,7 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
   }
   c.raster_out = par_has(par, "raster_out");
   c.voutput = par_has(par, "voutput");
+  c.voutput_stage = par_has(par, "voutput_stage");
   c.qoutput = par_has(par, "qoutput");
   c.elevoff = par_has(par, "elevoff");
   c.depthoff = par_has(par, "depthoff");

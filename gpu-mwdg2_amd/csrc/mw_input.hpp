@@ -55,6 +55,7 @@ struct HostCase {
   std::string dirroot = "res", resroot = "res";   // classes/PlottingParams.h:17-45
   // raster maps every saveint seconds (classes/PlottingParams.h:25-33)
   bool raster_out = false, voutput = false, qoutput = false, elevoff = false, depthoff = false;
+  bool voutput_stage = false;     // gauge velocities to .xvelocity / .yvelocity
   real saveint = real(9999.0);
   int saveint_count0 = 9999;
   HostBoundary<real> bdy[4];          // N, E, S, W

@@ -49,6 +49,7 @@ struct Params {
   int btype[4];                   // N, E, S, W boundary type
   int bstart[4], bend[4];
   int monai;                      // west HFIX/HVAR is the Monai non-reflective wavemaker
+  int oregon;                     // west HFIX/HVAR is the Oregon Seaside wavemaker (if not monai)
   real massint;                   // stage-output interval
 };
 

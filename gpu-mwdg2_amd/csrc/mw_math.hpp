@@ -89,6 +89,41 @@ template <> MW_HD float cbrt_newton<float>(float x) {
   return sub ? y / 256.0f : y;
 }
 
+// e^x by Cody-Waite range reduction, x = n ln2 + r with |r| <= ln2 / 2 and ln2
+// split into a head with trailing zero bits (so n * head is exact) and a tail
+// (W. J. Cody and W. Waite, Software Manual for the Elementary Functions,
+// 1980), then the Taylor series of e^r to degree 13 (FP64) or 8 (FP32) in
+// nested form 1 + r (1 + r/2 (1 + r/3 (...))), scaled by 2^n through exponent
+// bits. Used for the Oregon wavemaker pulse in place of exp, so host and device
+// give identical bits; within 2 ulp of the C library's exp.
+template <typename real> MW_HD real exp_cw(real x);
+template <> MW_HD double exp_cw<double>(double x) {
+  if (x != x) return x;
+  if (x > 709.8) return __builtin_inf();
+  if (x < -745.2) return 0.0;
+  const double n = __builtin_rint(x * 1.4426950408889634);
+  const double r = (x - n * 0.693147180369123816490) - n * 1.90821492927058770002e-10;
+  double p = 1.0;
+  for (int k = 13; k >= 1; --k) p = 1.0 + r * p / (double)k;
+  const int e = (int)n;
+  if (e > 1023) return (p * __builtin_bit_cast(double, (uint64_t)(e - 1 + 1023) << 52)) * 2.0;
+  if (e >= -1022) return p * __builtin_bit_cast(double, (uint64_t)(e + 1023) << 52);
+  return (p * __builtin_bit_cast(double, (uint64_t)(e + 54 + 1023) << 52)) * 5.551115123125783e-17;   // 2^-54
+}
+template <> MW_HD float exp_cw<float>(float x) {
+  if (x != x) return x;
+  if (x > 88.73f) return __builtin_inff();
+  if (x < -103.98f) return 0.0f;
+  const float n = __builtin_rintf(x * 1.44269504f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.428606765330187045e-06f;
+  float p = 1.0f;
+  for (int k = 8; k >= 1; --k) p = 1.0f + r * p / (float)k;
+  const int e = (int)n;
+  if (e > 127) return (p * __builtin_bit_cast(float, (uint32_t)(e - 1 + 127) << 23)) * 2.0f;
+  if (e >= -126) return p * __builtin_bit_cast(float, (uint32_t)(e + 127) << 23);
+  return (p * __builtin_bit_cast(float, (uint32_t)(e + 25 + 127) << 23)) * 2.98023223876953125e-08f;   // 2^-25
+}
+
 // ---------------------------------------------------------------------------
 // Multiwavelet filter bank. The analysis map of one variable is a 12x12
 // orthogonal matrix acting on the 4 children's (0, 1x, 1y) coefficients; its

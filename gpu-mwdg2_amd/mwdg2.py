@@ -173,10 +173,11 @@ class Simulation:
         _check(lib().mw_set_initial(self._h, ptrs))
 
     def stage_records(self) -> np.ndarray:
-        """Rows of (time, depth at each gauge, bed elevation at each gauge)."""
+        """Rows of (time, then at each gauge: depth, bed elevation, x velocity,
+        y velocity), each group n_points wide."""
         rows, npt = ctypes.c_int(), ctypes.c_int()
         _check(lib().mw_stage_records(self._h, None, 0, ctypes.byref(rows), ctypes.byref(npt)))
-        out = np.zeros((max(rows.value, 1), 1 + 2 * npt.value))
+        out = np.zeros((max(rows.value, 1), 1 + 4 * npt.value))
         if rows.value:
             _check(lib().mw_stage_records(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), rows.value,
                                           ctypes.byref(rows), ctypes.byref(npt)))

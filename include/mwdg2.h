@@ -154,13 +154,16 @@ int mw_filter_table(int precision, void* out144);
 
 /* Stage (gauge) output recorded on the device at the end of every step whose
  * time reaches massint * count (run_simulation.cu:707-750,
- * classes/SaveInterval.h:47-56): rows of 1 + 2 * n_points doubles: time, the
- * depth h0 of the leaf holding each gauge's finest cell, then that leaf's z0
- * (output/project_assem_sol.cu:60-77). *n_rows receives the number of rows
- * recorded; at most max_rows are copied. */
+ * classes/SaveInterval.h:47-56): rows of 1 + 4 * n_points doubles: time, the
+ * depth h0 = eta0 - z0 of the leaf holding each gauge's finest cell, that
+ * leaf's z0 (output/project_assem_sol.cu:60-77), then its x and y velocities
+ * qx0 / h0 and qy0 / h0, 0 below tol_h (output/write_stage_point_data.cu:159-160).
+ * *n_rows receives the number of rows recorded; at most max_rows are copied. */
 int mw_stage_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows, int* n_points);
 /* Writes the recorded rows as the reference's stage file
- * (output/write_stage_point_data.cu:37-190); path NULL = dirroot/resroot.stage. */
+ * (output/write_stage_point_data.cu:37-190); path NULL = dirroot/resroot.stage.
+ * With the .par keyword voutput_stage, also <path minus .stage>.xvelocity and
+ * .yvelocity in the same layout. */
 int mw_write_stage_file(mw_ctx* ctx, const char* path);
 
 /* Raster maps of the current leaf solution on the DEM's cells, written as

@@ -264,6 +264,8 @@ int mwo_compact(int code) { return compact(code); }
 int mwo_lvl_idx(int level) { return lvl_idx(level); }
 double mwo_cbrt_d(double x) { return cbrt_newton<double>(x); }
 float mwo_cbrt_f(float x) { return cbrt_newton<float>(x); }
+double mwo_exp_d(double x) { return exp_cw<double>(x); }
+float mwo_exp_f(float x) { return exp_cw<float>(x); }
 
 // children[4][3] (child c, mode m) -> out[12] = s0,s1x,s1y, a0,a1x,a1y, b.., g..
 void mwo_encode_mw_d(const double* ch, double* out) {

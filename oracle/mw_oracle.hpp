@@ -100,6 +100,41 @@ template <> inline float cbrt_newton<float>(float x) {
   return sub ? y / 256.0f : y;                      // 2^8 = cbrt(2^24)
 }
 
+// e^x for the Oregon wavemaker pulse: Cody-Waite reduction x = n ln2 + r
+// (W. J. Cody and W. Waite, Software Manual for the Elementary Functions,
+// 1980) with ln2 = head + tail, head with trailing zero bits so n * head is
+// exact; e^r by its Taylor series to degree 13 (FP64) / 8 (FP32) in nested
+// form; 2^n through exponent bits (two factors near overflow and underflow).
+template <typename real> inline real exp_cw(real x);
+template <> inline double exp_cw<double>(double x) {
+  if (std::isnan(x)) return x;
+  if (x > 709.8) return INFINITY;
+  if (x < -745.2) return 0.0;
+  const double n = std::rint(x * 1.4426950408889634);                 // x / ln2
+  const double r = (x - n * 0.693147180369123816490) - n * 1.90821492927058770002e-10;
+  double p = 1.0;
+  for (int k = 13; k >= 1; --k) p = 1.0 + r * p / (double)k;
+  auto pow2 = [](int e) { const uint64_t b = (uint64_t)(e + 1023) << 52; double v; std::memcpy(&v, &b, 8); return v; };
+  const int e = (int)n;
+  if (e > 1023) return (p * pow2(e - 1)) * 2.0;
+  if (e >= -1022) return p * pow2(e);
+  return (p * pow2(e + 54)) * pow2(-54);
+}
+template <> inline float exp_cw<float>(float x) {
+  if (std::isnan(x)) return x;
+  if (x > 88.73f) return INFINITY;
+  if (x < -103.98f) return 0.0f;
+  const float n = std::rint(x * 1.44269504f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.428606765330187045e-06f;
+  float p = 1.0f;
+  for (int k = 8; k >= 1; --k) p = 1.0f + r * p / (float)k;
+  auto pow2 = [](int e) { const uint32_t b = (uint32_t)(e + 127) << 23; float v; std::memcpy(&v, &b, 4); return v; };
+  const int e = (int)n;
+  if (e > 127) return (p * pow2(e - 1)) * 2.0f;
+  if (e >= -126) return p * pow2(e);
+  return (p * pow2(e + 25)) * pow2(-25);
+}
+
 // ---------------------------------------------------------------------------
 // Input parsing: a restatement of the keyword readers (prefix match of the
 // first token, input/read_keyword_real.cpp:30-40, read_keyword_int.cpp,

@@ -49,6 +49,10 @@ def lib():
         L.mwo_cbrt_d.argtypes = [d]
         L.mwo_cbrt_f.restype = ctypes.c_float
         L.mwo_cbrt_f.argtypes = [ctypes.c_float]
+        L.mwo_exp_d.restype = d
+        L.mwo_exp_d.argtypes = [d]
+        L.mwo_exp_f.restype = ctypes.c_float
+        L.mwo_exp_f.argtypes = [ctypes.c_float]
         for n in ("mwo_encode_mw_d", "mwo_decode_mw_d", "mwo_encode_hw_d", "mwo_decode_hw_d",
                   "mwo_encode_mw_f", "mwo_decode_mw_f"):
             getattr(L, n).argtypes = [vp, vp]
@@ -194,6 +198,10 @@ def wavemaker(h_in, dt, dxl, hp, up, g=9.80665):
 
 def cbrt(x, dtype=np.float64):
     return lib().mwo_cbrt_d(float(x)) if dtype == np.float64 else lib().mwo_cbrt_f(float(x))
+
+
+def exp(x, dtype=np.float64):
+    return lib().mwo_exp_d(float(x)) if dtype == np.float64 else lib().mwo_exp_f(float(x))
 
 
 def morton(x: int, y: int) -> int:

@@ -64,7 +64,10 @@ def compare_step(o: oracle.Oracle, s, mw: bool, tol: float = 0.0, label: str = "
     return rep
 
 
-def run_pair(par: str, precision: str, steps: int, tol: float = 0.0, check_every: int = 1):
+def run_pair(par: str, precision: str, steps: int, tol: float = 0.0, check_every: int = 1,
+             t0: float | None = None):
+    """Steps both and compares after every check_every steps; t0 sets the
+    start time of both (for time-dependent boundaries)."""
     import mwdg2
     o = oracle.Oracle(par, precision)
     s = mwdg2.Simulation(par, precision)
@@ -80,6 +83,9 @@ def run_pair(par: str, precision: str, steps: int, tol: float = 0.0, check_every
         for name in ("z0",) + (("z1x", "z1y") if mw else ()):
             assert np.array_equal(o.get(name), s.get(name)), f"z hierarchy {name} differs"
         assert np.array_equal(o.get("preflag")[3: int(o.scalar("D"))], s.get("preflag")[3: int(o.scalar("D"))]), "preflags differ"
+        if t0 is not None:
+            o.set_scalar("time", t0)
+            s.set_scalar("time", t0)
         reps = []
         for k in range(steps):
             more = o.step()

@@ -97,3 +97,50 @@ def test_raster_maps_match_leaf_solution(case_dir, solver):
         got = data[::-1, :]                     # first row written is the northern one
         assert got.shape == (ysz, xsz)
         assert np.allclose(got, exp[ext], rtol=2e-6, atol=2e-7), (ext, np.max(np.abs(got - exp[ext])))
+
+
+def read_stage_file(path):
+    lines = open(path).read().splitlines()
+    head = lines[0]
+    i = lines.index("Stage information (stage,x,y,elev):")
+    j = next(k for k, l in enumerate(lines) if l.startswith("Time; stages"))
+    gauges = np.array([[float(v) for v in l.split()] for l in lines[i + 1:j - 2]])
+    rows = np.array([[float(v) for v in l.split()] for l in lines[j + 1:]])
+    return head, gauges, rows
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_stage_velocity_files(case_dir, solver):
+    """voutput_stage (output/write_stage_point_data.cu:26-190): the .stage,
+    .xvelocity and .yvelocity files, each with the gauge table and one row per
+    save, against the oracle's gauge rows (depth, z, qx / h, qy / h, 0 below
+    tol_h) bit for bit in the records and to the printed digits in the files."""
+    import mwdg2
+    d = os.path.join(str(case_dir), f"vstage_{solver}")
+    par = cases.random_case(d, L=6, seed=31, solver=solver, manning=0.02, t_end=1.0, stage=4, startq=True)
+    txt = open(par).read().replace("massint       1.0", "massint       0.1")
+    with open(par, "w") as f:
+        f.write(txt + "voutput_stage\n")
+    o = oracle.Oracle(par, "f64")
+    while o.step():
+        pass
+    with mwdg2.Simulation(par, "f64") as s:
+        s.run(max_steps=100000, batch=16)
+        rows = s.stage_records()
+        out = os.path.join(d, "res", "gauges.stage")
+        s.write_stage_file(out)
+    ro = o.get("stage").reshape(-1, rows.shape[1])
+    o.close()
+    npt = (rows.shape[1] - 1) // 4
+    assert npt == 4 and rows.shape[0] >= 8
+    assert np.array_equal(rows, ro)
+    assert np.any(rows[:, 1 + 2 * npt:] != 0.0)
+    for ext, col, title in ((".stage", 0, "depth (m)"), (".xvelocity", 2, "x velocity (m/s)"),
+                            (".yvelocity", 3, "y velocity (m/s)")):
+        head, gauges, got = read_stage_file(os.path.join(d, "res", "gauges" + ext))
+        assert head == f"Stage output, {title}."
+        assert gauges.shape == (npt, 4) and np.array_equal(gauges[:, 0], np.arange(1, npt + 1))
+        assert np.allclose(gauges[:, 3], rows[0, 1 + npt:1 + 2 * npt], atol=6e-7)
+        assert got.shape == (rows.shape[0], 1 + npt)
+        assert np.allclose(got[:, 0], rows[:, 0], atol=6e-7)
+        assert np.allclose(got[:, 1:], rows[:, 1 + col * npt:1 + (col + 1) * npt], atol=6e-7)

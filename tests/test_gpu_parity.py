@@ -133,3 +133,18 @@ def test_monai_wavemaker_and_stage_rows(case_dir, solver):
     o.close()
     assert rows.shape[0] >= 8
     assert np.array_equal(rows, ro)
+
+
+@pytest.mark.parametrize("solver,prec", [("mwdg2", "f64"), ("hwfv1", "f64"), ("mwdg2", "f32")])
+def test_oregon_wavemaker(case_dir, solver, prec):
+    """West HVAR boundary as the Oregon Seaside wavemaker
+    (add_ghost_cells.cu:471-481), stepped through the pulse's peak
+    (t = 14.75 s), against the oracle bit for bit."""
+    bci = "W 0 64 HVAR wave\n"
+    bdy = "wave\n2 seconds\n2.5 0\n2.5 100\n"
+    par = _mk(case_dir, f"oregon_{solver}_{prec}", cases.random_case, L=6, seed=29, solver=solver, manning=0.02,
+              t_end=30.0, bci=bci, bdy=bdy, wet_frac=0.8)
+    with open(par, "a") as f:
+        f.write("oregon-seaside\n")
+    reps = parity_util.run_pair(par, prec, steps=30, t0=14.6)
+    assert len(reps) == 30

@@ -212,3 +212,68 @@ def test_wavemaker_keeps_still_water_still():
     # a cell below the still depth draws water in through the boundary
     h2, u2 = oracle.wavemaker(0.0, 0.01, 0.014, 0.12, 0.0)
     assert 0.12 < h2 < hb and u2 > 0.0
+
+
+def test_exp_within_two_ulp_of_libm():
+    """The oracle's exp (Cody-Waite reduction + Taylor series, used for the
+    Oregon wavemaker pulse) against numpy's exp, FP64 and FP32, over the
+    pulse's range, the whole finite range and the reduction's edges."""
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-60.0, 0.0, 4000), rng.uniform(-745.0, 709.7, 4000),
+                         np.linspace(-0.35, 0.35, 301), [0.0, 1e-12, -1e-12, 0.5 * np.log(2.0), -708.5, -744.0]])
+    for x in xs:
+        e, r = oracle.exp(float(x)), float(np.exp(x))
+        if r < 2.3e-308:                                   # subnormal results: absolute half-ulp steps
+            assert abs(e - r) <= 2 * 5e-324
+        else:
+            assert abs(e - r) <= 2 * np.spacing(r), (x, e, r)
+    assert oracle.exp(0.0) == 1.0 and oracle.exp(-800.0) == 0.0 and np.isinf(oracle.exp(710.0))
+    xf = np.concatenate([rng.uniform(-60.0, 0.0, 4000), rng.uniform(-87.0, 88.0, 4000)]).astype(np.float32)
+    for x in xf:
+        e, r = np.float32(oracle.exp(float(x), np.float32)), np.exp(np.float64(x))
+        assert abs(float(e) - r) <= 2 * float(np.spacing(np.float32(r))), (x, e, r)
+    assert oracle.exp(0.0, np.float32) == 1.0
+
+
+def test_oregon_ghost_closed_form(tmp_path):
+    """Oregon Seaside wavemaker (neighbours/add_ghost_cells.cu:471-481): at
+    the finest west-border leaves of an HVAR span the ghost keeps the local
+    depth and carries the discharge (2 s(t) - v) h with
+    s(t) = 0.6 exp(-(t - 14.75)^2 / 4) and v = qx / h (0 below tol_h); the
+    local depth is not reset to the inlet as for a plain HVAR boundary."""
+    bci = "W 0 16 HVAR wave\n"
+    bdy = "wave\n2 seconds\n5.0 0\n5.0 100\n"
+    par = cases.random_case(str(tmp_path / "oregon"), L=4, seed=5, eps=0.0, t_end=30.0, bci=bci, bdy=bdy,
+                            startq=True, wet_frac=0.8)
+    with open(par, "a") as f:
+        f.write("oregon-seaside\n")
+    o = oracle.Oracle(par, "f64")
+    try:
+        o.step()
+        n = int(o.scalar("length"))
+        act = o.get("assem.act")[:n].astype(np.int64)
+        lev = o.get("assem.lev")[:n].astype(np.int64)
+        L = int(o.scalar("L"))
+        code = act - np.array([oracle.lvl_idx(int(l)) for l in lev])
+        x = np.array([oracle.compact(int(c)) for c in code])
+        west = (x == 0) & (lev == L)
+        assert west.sum() == 16
+        for t in (10.0, 14.0, 14.75, 16.3):
+            o.set_scalar("time", t)
+            o.stage("load_soln")
+            h = o.get("assem.h0")[:n].copy()
+            qx = o.get("assem.qx0")[:n].copy()
+            o.stage("ghost")
+            assert np.array_equal(o.get("assem.h0")[:n], h)
+            s = 0.6 * np.exp(-0.25 * (t - 14.75) ** 2)
+            v = np.where(h > 1e-3, qx / np.where(h > 0, h, 1.0), 0.0)
+            gh, gq = o.get("nb3.h0")[:n], o.get("nb3.qx0")[:n]
+            assert np.array_equal(gh[west], h[west])
+            assert np.allclose(gq[west], (2.0 * s - v[west]) * h[west], rtol=1e-13, atol=1e-15)
+            assert np.any(h[west] > 1e-3)
+            # the other west-border ghosts (coarser leaves: none here) and the
+            # east border are closed walls: zero normal discharge
+            east = o.get("nb1.act")[:n] == -1
+            assert np.all(o.get("nb1.qx0")[:n][east] == 0.0)
+    finally:
+        o.close()
