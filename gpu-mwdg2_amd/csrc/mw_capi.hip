@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -117,6 +118,13 @@ struct CtxBase {
   virtual double dx() const = 0;
   virtual std::string out_path() const = 0;
   virtual bool voutput_stage() const = 0;
+  virtual int cumulative_records(double* out, int max_rows) = 0;
+  virtual void write_cumulative(const std::string& path) = 0;
+  virtual std::string cumulative_path() const = 0;
+  virtual void write_vtk(int index) = 0;
+  virtual void write_c_prop() = 0;
+  virtual void save_outputs(int index) = 0;
+  virtual bool saveint_outputs() const = 0;
 };
 
 template <typename real>
@@ -145,8 +153,14 @@ struct Ctx : CtxBase {
   WetList wet[2]{};                           // wet-leaf lists of the two MWDG2 stages
   int n_stage = 0;                            // gauges (stage points)
   int* stage_codes = nullptr;
-  real* rec = nullptr;                        // stage-output rows: time, depths, elevations
+  real* rec = nullptr;                        // stage-output rows: time, depths, elevations, velocities
+  double* cum = nullptr;                      // cumulative rows: time, dt, leaves, mra ticks, ticks, step
   int rec_cap = 0;
+  real* leafdt = nullptr;                     // per-leaf CFL time step (VTK output)
+  real* lbuf = nullptr;                       // leaf-field staging for the leaf-list writers
+  double* clk = nullptr;                      // k_clock output
+  int cprop_writes = 0;
+  std::vector<std::array<double, 6>> cprop_rows;   // cumulative rows written with the c-property files
   unsigned long long* cnt[3] = {nullptr, nullptr, nullptr};  // P_e, P_d, rescaled parents (slot arrays)
   // series and sources on the device
   int *s_off = nullptr, *s_len = nullptr;
@@ -241,14 +255,22 @@ struct Ctx : CtxBase {
     }
     if (hc.limitslopes && mwm) uprev = alloc<real>((size_t)9 * N);
     n_stage = (int)hc.stage_codes.size();
-    if (n_stage > 0 && hc.massint_count0 == 0 && hc.massint > real(0)) {
+    if ((n_stage > 0 || hc.cumulative) && hc.massint_count0 == 0 && hc.massint > real(0)) {
       const double rows = std::floor((double)hc.sim_time / (double)hc.massint) + 3.0;
       rec_cap = (int)std::min(rows, (double)(1 << 20));
-      rec = alloc<real>((size_t)rec_cap * (1 + 4 * n_stage));
-      stage_codes = alloc<int>(n_stage);
-      MW_HIP(hipMemcpyAsync(stage_codes, hc.stage_codes.data(), sizeof(int) * n_stage, hipMemcpyHostToDevice, stream));
+      if (n_stage > 0) {
+        rec = alloc<real>((size_t)rec_cap * (1 + 4 * n_stage));
+        stage_codes = alloc<int>(n_stage);
+        MW_HIP(hipMemcpyAsync(stage_codes, hc.stage_codes.data(), sizeof(int) * n_stage, hipMemcpyHostToDevice,
+                              stream));
+      }
+      if (hc.cumulative) cum = alloc<double>((size_t)rec_cap * 6);
     }
     fill_params();
+    if (hc.vtk) {
+      leafdt = alloc<real>(N);
+      P.leaf_dt = leafdt;
+    }
     upload_sources_and_series();
     build_initial();
   }
@@ -570,7 +592,8 @@ struct Ctx : CtxBase {
     int* done = &st->done;
     int* nl = &st->n_leaves;
     timed("emit_count", [&] { hipLaunchKernelGGL(k_emit_count, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums); });
-    timed("emit_scan", [&] { hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles); });
+    timed("emit_scan", [&] { hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles,
+                                                  &st->clk_step, &st->mra_ticks); });
     if (sweep) {
       WetList W = wet[0];
       W.cap = 4 * EMIT_TILE;
@@ -681,14 +704,14 @@ struct Ctx : CtxBase {
       if (hc.epsilon > real(0.0)) encode_scale_launch();
       stage2_launch();
     }
-    if (rec) {
+    if (rec || cum) {
       timed("stage_record", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_stage_record<real, true>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh,
-                             leaflev, stage_codes, n_stage, rec, rec_cap);
+          hipLaunchKernelGGL((k_stage_record<real, true>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
+                             leaflev, stage_codes, n_stage, rec, cum, rec_cap, red, 2 * leaf_grid());
         else
-          hipLaunchKernelGGL((k_stage_record<real, false>), dim3(1), dim3(64), 0, stream, st, P, hier[main_h], zh,
-                             leaflev, stage_codes, n_stage, rec, rec_cap);
+          hipLaunchKernelGGL((k_stage_record<real, false>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
+                             leaflev, stage_codes, n_stage, rec, cum, rec_cap, red, 2 * leaf_grid());
       });
     }
     issued_first = true;
@@ -760,6 +783,177 @@ struct Ctx : CtxBase {
     }
     return false;
   }
+  // cumulative rows of the massint saves (device) and of the c-property saves
+  // (host), as (time, dt, leaves, mra ticks, ticks, step, kind 0 c_prop / 1 massint)
+  std::vector<std::array<double, 7>> cumulative_rows() {
+    std::vector<std::array<double, 7>> out;
+    for (const auto& r : cprop_rows) out.push_back({r[0], r[1], r[2], r[3], r[4], r[5], 0.0});
+    if (cum) {
+      const DevState<real> s = read_state();
+      const int rows = std::min(s.n_records, rec_cap);
+      std::vector<double> v((size_t)rows * 6);
+      if (rows > 0) {
+        MW_HIP(hipMemcpyAsync(v.data(), cum, sizeof(double) * v.size(), hipMemcpyDeviceToHost, stream));
+        MW_HIP(hipStreamSynchronize(stream));
+      }
+      for (int i = 0; i < rows; ++i) {
+        const double* r = v.data() + (size_t)i * 6;
+        out.push_back({r[0], r[1], r[2], r[3], r[4], r[5], 1.0});
+      }
+    }
+    // file order: per step the saveint block, then the massint block
+    // (run_simulation.cu:656-750); both open the file with "w" on the first
+    // step, so a first-step massint row replaces the c-property one
+    std::stable_sort(out.begin(), out.end(), [](const auto& a, const auto& b) {
+      return a[5] != b[5] ? a[5] < b[5] : a[6] < b[6];
+    });
+    const double first = out.empty() ? 0.0 : out.front()[5];
+    bool massint_first = false;
+    for (const auto& r : out) massint_first |= (r[5] == first && r[6] == 1.0);
+    if (massint_first)
+      out.erase(std::remove_if(out.begin(), out.end(), [&](const auto& r) { return r[5] == first && r[6] == 0.0; }),
+                out.end());
+    return out;
+  }
+  int cumulative_records(double* out, int max_rows) override {
+    const auto rows = cumulative_rows();
+    if (out)
+      for (int i = 0; i < std::min<int>(max_rows, (int)rows.size()); ++i)
+        for (int k = 0; k < 7; ++k) out[(size_t)i * 7 + k] = rows[i][k];
+    return (int)rows.size();
+  }
+  double clock_hz() {
+    int dev = 0, khz = 0;
+    MW_HIP(hipGetDevice(&dev));
+    MW_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    return 1000.0 * (double)khz;
+  }
+  // output/write_cumulative_data.cpp:3-50
+  void write_cumulative(const std::string& path) override {
+    const auto rows = cumulative_rows();
+    if (rows.empty()) return;
+    const double hz = clock_hz();
+    mkdir(hc.dirroot.c_str(), 0755);
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw InputError("cannot write " + path);
+    std::fprintf(f, "simtime,runtime_mra,runtime_solver,runtime_total,dt,reduction\n");
+    for (const auto& r : rows) {
+      const real reduction = (real(100.0) - real(100.0) * real((int)r[2]) / real(hc.xsz * hc.ysz)) / real(100.0);
+      const double mra = r[3] / hz, total = r[4] / hz;
+      std::fprintf(f, "%f,%f,%f,%f,%f,%f\n", r[0], mra, total - mra, total, r[1], (double)reduction);
+    }
+    std::fclose(f);
+  }
+  std::string cumulative_path() const override { return hc.dirroot + "/" + hc.resroot + ".cumu"; }
+
+  // leaf values (field ids of k_leaf_fields) of the current leaf list, field-major
+  std::vector<real> leaf_fields(const std::vector<int>& ids, int n) {
+    if (!lbuf) lbuf = alloc<real>((size_t)8 * N);
+    FieldSel F{};
+    F.n = (int)ids.size();
+    for (int k = 0; k < F.n; ++k) F.v[k] = ids[k];
+    if (mwm)
+      hipLaunchKernelGGL((k_leaf_fields<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
+                         F, lbuf, N);
+    else
+      hipLaunchKernelGGL((k_leaf_fields<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
+                         F, lbuf, N);
+    MW_HIP(hipGetLastError());
+    std::vector<real> out((size_t)F.n * n);
+    for (int k = 0; k < F.n; ++k)
+      MW_HIP(hipMemcpyAsync(out.data() + (size_t)k * n, lbuf + (size_t)k * N, sizeof(real) * n, hipMemcpyDeviceToHost,
+                            stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    return out;
+  }
+
+  // output/write_soln_vtk.cu:3-221: dirroot/resroot-<index>.vtk, one pixel per leaf
+  void write_vtk(int index) override {
+    if (!leafdt) throw InputError("the VTK output needs the vtk keyword in the .par file");
+    const DevState<real> s = read_state();
+    const int n = s.n_leaves;
+    const std::vector<real> v = leaf_fields({0, 9, 3, 6}, n);
+    std::vector<int> act(n);
+    std::vector<real> dtl(n);
+    MW_HIP(hipMemcpyAsync(act.data(), leaf, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipMemcpyAsync(dtl.data(), leafdt, sizeof(real) * n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    mkdir(hc.dirroot.c_str(), 0755);
+    const std::string path = hc.dirroot + "/" + hc.resroot + "-" + std::to_string(index) + ".vtk";
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw InputError("cannot write " + path);
+    std::fprintf(f, "# vtk DataFile Version 3.0\nMultiresolution flow and topo data\nASCII\n\n"
+                    "DATASET UNSTRUCTURED_GRID\nPOINTS %d float\n", n * 4);
+    for (int i = 0; i < n; ++i) {
+      const int level = level_of(act[i]);
+      const int code = act[i] - lvl_idx(level);
+      const real cs = hc.dx * real(1 << (L - level));
+      const int x = compact(code), y = compact(code >> 1);
+      const real x0 = hc.xmin + real(x) * cs, x1 = hc.xmin + real(x + 1) * cs;
+      const real y0 = hc.ymin + real(y) * cs, y1 = hc.ymin + real(y + 1) * cs;
+      std::fprintf(f, "%f %f %f\n%f %f %f\n%f %f %f\n%f %f %f\n", (double)x0, (double)y0, 1.0, (double)x0,
+                   (double)y1, 1.0, (double)x1, (double)y0, 1.0, (double)x1, (double)y1, 1.0);
+    }
+    std::fprintf(f, "\nCELLS %d %d\n", n, n * 5);
+    for (int i = 0; i < n; ++i) std::fprintf(f, "4 %d %d %d %d\n", 4 * i, 4 * i + 1, 4 * i + 2, 4 * i + 3);
+    std::fprintf(f, "\nCELL_TYPES %d\n", n);
+    for (int i = 0; i < n; ++i) std::fprintf(f, "8\n");
+    std::fprintf(f, "\nCELL_DATA %d\n", n);
+    const real* eta = v.data();
+    const real* z = v.data() + (size_t)n;
+    const real* qx = v.data() + (size_t)2 * n;
+    const real* qy = v.data() + (size_t)3 * n;
+    auto scalars = [&](const char* name, auto value) {
+      std::fprintf(f, "\nSCALARS %s float 1\nLOOKUP_TABLE default\n", name);
+      for (int i = 0; i < n; ++i) std::fprintf(f, "%f\n", (double)value(i));
+    };
+    scalars("dt", [&](int i) { return dtl[i]; });
+    scalars("h", [&](int i) { return eta[i] - z[i]; });
+    scalars("qx", [&](int i) { return qx[i]; });
+    scalars("qy", [&](int i) { return qy[i]; });
+    scalars("z", [&](int i) { return z[i]; });
+    scalars("eta", [&](int i) { return (eta[i] - z[i]) + z[i]; });
+    std::fclose(f);
+  }
+
+  // output/write_c_prop_data.cu:3-84 (+ append_reals_to_file.cu): one row of
+  // the leaves' discharges per save in <dirroot>qx0-c-prop.csv ... (the
+  // reference joins dirroot and the name without a separator), and a row of
+  // the cumulative output
+  void write_c_prop() override {
+    const DevState<real> s = read_state();
+    const int n = s.n_leaves;
+    const std::vector<int> ids = mwm ? std::vector<int>{3, 6, 4, 5, 7, 8} : std::vector<int>{3, 6};
+    const char* names[6] = {"qx0-c-prop", "qy0-c-prop", "qx1x-c-prop", "qx1y-c-prop", "qy1x-c-prop", "qy1y-c-prop"};
+    const std::vector<real> v = leaf_fields(ids, n);
+    const char* fmt = sizeof(real) == 4 ? "%.8f," : "%.15f,";
+    mkdir(hc.dirroot.c_str(), 0755);                  // classes/PlottingParams.h:48-53
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const std::string path = hc.dirroot + names[k] + ".csv";
+      FILE* f = std::fopen(path.c_str(), cprop_writes == 0 ? "w" : "a");
+      if (!f) throw InputError("cannot write " + path);
+      for (int i = 0; i < n; ++i) std::fprintf(f, fmt, (double)v[k * (size_t)n + i]);
+      std::fprintf(f, "\n");
+      std::fclose(f);
+    }
+    ++cprop_writes;
+    if (!clk) clk = alloc<double>(2);
+    hipLaunchKernelGGL((k_clock<real>), dim3(1), dim3(64), 0, stream, st, clk);
+    MW_HIP(hipGetLastError());
+    double c[2];
+    MW_HIP(hipMemcpyAsync(c, clk, sizeof c, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    cprop_rows.push_back({(double)s.time, (double)next_dt(), (double)n, c[0], c[1], (double)s.steps});
+  }
+
+  // the saveint block of run_simulation.cu:656-705: whatever the .par asks for
+  void save_outputs(int index) override {
+    if (hc.vtk) write_vtk(index);
+    if (hc.raster_out) write_rasters(index);
+    if (hc.c_prop) write_c_prop();
+  }
+  bool saveint_outputs() const override { return hc.vtk || hc.raster_out || hc.c_prop; }
+
   std::string out_path() const override { return hc.dirroot + "/" + hc.resroot + ".stage"; }
   bool voutput_stage() const override { return hc.voutput_stage; }
 
@@ -870,6 +1064,8 @@ struct Ctx : CtxBase {
       len = s.n_leaves;
       esz = sizeof(int);
       dev = (name == "leaves") ? (void*)leaf : (void*)nbr[name[3] - '0'];
+    } else if (name == "leaf_dt" && leafdt) {
+      len = read_state().n_leaves; esz = sizeof(real); dev = leafdt;
     } else if (name == "rev_z_order") { dev = rev_z; len = N; esz = sizeof(int); }
     else if (name == "rev_row_major") { dev = rev_rm; len = N; esz = sizeof(int); }
     if (!dev) throw std::invalid_argument("unknown or absent array " + name);
@@ -1275,6 +1471,31 @@ int mw_write_rasters(mw_ctx* ctx, int index) {
   return guarded([&] { ctx->impl->write_rasters(index); });
 }
 
+int mw_write_vtk(mw_ctx* ctx, int index) {
+  MW_NEED(ctx && index >= 0);
+  return guarded([&] { ctx->impl->write_vtk(index); });
+}
+
+int mw_write_c_prop(mw_ctx* ctx) {
+  MW_NEED(ctx);
+  return guarded([&] { ctx->impl->write_c_prop(); });
+}
+
+int mw_save_outputs(mw_ctx* ctx, int index) {
+  MW_NEED(ctx && index >= 0);
+  return guarded([&] { ctx->impl->save_outputs(index); });
+}
+
+int mw_cumulative_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows) {
+  MW_NEED(ctx && n_rows && max_rows >= 0 && (out || max_rows == 0));
+  return guarded([&] { *n_rows = ctx->impl->cumulative_records(out, max_rows); });
+}
+
+int mw_write_cumulative_file(mw_ctx* ctx, const char* path) {
+  MW_NEED(ctx);
+  return guarded([&] { ctx->impl->write_cumulative(path ? std::string(path) : ctx->impl->cumulative_path()); });
+}
+
 int mw_run_simulation(int argc, const char* const* argv) {
   MW_NEED(argc >= 2 && argv);
   const char* par = argv[argc - 1];
@@ -1288,15 +1509,15 @@ int mw_run_simulation(int argc, const char* const* argv) {
   mw_counters_get(ctx, &c);
   double n = (double)(1 << c.max_ref_lvl);
   long long last_print = -1;
-  const bool rasters = mw::par_has(par, "raster_out");
+  const bool saves = ctx->impl->saveint_outputs();
   int saved = 0;
   while (true) {
-    if (rasters) {          // the save check needs the time after every step (run_simulation.cu:656-690)
+    if (saves) {            // the save check needs the time after every step (run_simulation.cu:656-705)
       rc = mw_run_until(ctx, c.steps + 1, 1, &c);
       if (rc == MW_OK && !c.done) {
         bool due = false;
         rc = guarded([&] { due = ctx->impl->raster_due(); });
-        if (rc == MW_OK && due) rc = mw_write_rasters(ctx, saved++);
+        if (rc == MW_OK && due) rc = mw_save_outputs(ctx, saved++);
       }
     } else {
       rc = mw_run_until(ctx, c.steps + 10000, 64, &c);
@@ -1315,6 +1536,7 @@ int mw_run_simulation(int argc, const char* const* argv) {
                 100.0 - 100.0 * c.n_leaves / (n * n), c.dt_next, c.steps + 1, c.time);
     std::printf("Loop time: %f s\n", secs);
     if (mw_write_stage_file(ctx, nullptr) != MW_OK) std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
+    if (mw_write_cumulative_file(ctx, nullptr) != MW_OK) std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
   } else {
     std::fprintf(stderr, "mwdg2: %s\n", mw_last_error());
   }

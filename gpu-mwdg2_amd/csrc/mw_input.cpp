@@ -219,6 +219,9 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
   c.raster_out = par_has(par, "raster_out");
   c.voutput = par_has(par, "voutput");
   c.voutput_stage = par_has(par, "voutput_stage");
+  c.vtk = par_has(par, "vtk");
+  c.c_prop = par_has(par, "c_prop");
+  c.cumulative = par_has(par, "cumulative");
   c.qoutput = par_has(par, "qoutput");
   c.elevoff = par_has(par, "elevoff");
   c.depthoff = par_has(par, "depthoff");

@@ -56,6 +56,9 @@ struct HostCase {
   // raster maps every saveint seconds (classes/PlottingParams.h:25-33)
   bool raster_out = false, voutput = false, qoutput = false, elevoff = false, depthoff = false;
   bool voutput_stage = false;     // gauge velocities to .xvelocity / .yvelocity
+  bool vtk = false;               // leaf mesh and solution as VTK every saveint
+  bool c_prop = false;            // leaf discharges to <dirroot>q*-c-prop.csv every saveint
+  bool cumulative = false;        // .cumu rows (time, runtimes, dt, reduction) every massint
   real saveint = real(9999.0);
   int saveint_count0 = 9999;
   HostBoundary<real> bdy[4];          // N, E, S, W

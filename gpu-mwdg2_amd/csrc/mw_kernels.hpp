@@ -28,6 +28,9 @@ struct DevState {
   unsigned long long wet_sum[2];  // leaves of the wet lists of stage 1 and 2, summed over steps
   int save_count;                 // stage-output counter (classes/SaveInterval.h:47-56)
   int n_records;                  // stage-output rows written
+  unsigned long long clk0;        // device wall clock at the first step's start
+  unsigned long long clk_step;    // ... at this step's start
+  unsigned long long mra_ticks;   // summed step start -> leaf list (the reference's time_mra)
 };
 
 // Time series of the boundaries (entries 0..3) and point sources (4..).
@@ -51,6 +54,7 @@ struct Params {
   int monai;                      // west HFIX/HVAR is the Monai non-reflective wavemaker
   int oregon;                     // west HFIX/HVAR is the Oregon Seaside wavemaker (if not monai)
   real massint;                   // stage-output interval
+  real* leaf_dt;                  // per-leaf CFL time step of the last update (VTK output), or null
 };
 
 // Flow scale coefficients of every node of the hierarchy, eta form. Coefficient

@@ -85,6 +85,11 @@ def lib():
         L.mw_stage_records.argtypes = [vp, ctypes.POINTER(d), i, ctypes.POINTER(i), ctypes.POINTER(i)]
         L.mw_write_stage_file.argtypes = [vp, cp]
         L.mw_write_rasters.argtypes = [vp, i]
+        L.mw_write_vtk.argtypes = [vp, i]
+        L.mw_write_c_prop.argtypes = [vp]
+        L.mw_save_outputs.argtypes = [vp, i]
+        L.mw_cumulative_records.argtypes = [vp, ctypes.POINTER(d), i, ctypes.POINTER(i)]
+        L.mw_write_cumulative_file.argtypes = [vp, cp]
         _lib = L
     return _lib
 
@@ -188,6 +193,30 @@ class Simulation:
 
     def write_stage_file(self, path: str | None = None):
         _check(lib().mw_write_stage_file(self._h, path.encode() if path else None))
+
+    def write_vtk(self, index: int = 0):
+        _check(lib().mw_write_vtk(self._h, int(index)))
+
+    def write_c_prop(self):
+        _check(lib().mw_write_c_prop(self._h))
+
+    def save_outputs(self, index: int = 0):
+        _check(lib().mw_save_outputs(self._h, int(index)))
+
+    def cumulative_records(self) -> np.ndarray:
+        """Rows of (time, next dt, leaf count, step start -> leaf list device
+        clock ticks summed, device clock ticks since the first step, step,
+        0 c-property / 1 massint) in .cumu file order."""
+        rows = ctypes.c_int()
+        _check(lib().mw_cumulative_records(self._h, None, 0, ctypes.byref(rows)))
+        out = np.zeros((max(rows.value, 1), 7))
+        if rows.value:
+            _check(lib().mw_cumulative_records(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               rows.value, ctypes.byref(rows)))
+        return out[:rows.value]
+
+    def write_cumulative_file(self, path: str | None = None):
+        _check(lib().mw_write_cumulative_file(self._h, path.encode() if path else None))
 
     # profiling -----------------------------------------------------------
     def profile(self, level: int = 1):

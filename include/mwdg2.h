@@ -128,7 +128,9 @@ int mw_stage_named(mw_ctx* ctx, const char* name);
 
 /* Named device arrays: hierarchy "A.eta0".."A.qy1y", "B.*", "z0","z1x","z1y";
  * masks "sig" (current tree), "sig_next", "xflag", "leaflev", "preflag";
- * leaf lists "leaves", "nbr0".."nbr3" (N,E,S,W); "rev_z_order","rev_row_major".
+ * leaf lists "leaves", "nbr0".."nbr3" (N,E,S,W); "rev_z_order","rev_row_major";
+ * "leaf_dt" (each leaf's CFL time step from the last update, kept only with
+ * the .par keyword vtk; operators/dg2_update.cu:61,357, fv1_update.cu:51,237).
  * count is in elements; *len (optional) receives the array length. */
 int mw_download(mw_ctx* ctx, const char* name, void* host, long long count, long long* len);
 int mw_upload(mw_ctx* ctx, const char* name, const void* host, long long count);
@@ -171,6 +173,28 @@ int mw_write_stage_file(mw_ctx* ctx, const char* path);
  * .par's raster keywords (output/write_all_raster_maps_{dg2,fv1}.cu,
  * write_raster_file.cu:3-57); h is the stored eta minus z. */
 int mw_write_rasters(mw_ctx* ctx, int index);
+/* VTK file of the leaves (output/write_soln_vtk.cu:3-221):
+ * dirroot/resroot-<index>.vtk, one pixel cell per leaf with its CFL time step,
+ * h, qx, qy, z and eta. Needs the .par keyword vtk (the per-leaf time steps
+ * are kept only then). */
+int mw_write_vtk(mw_ctx* ctx, int index);
+/* c-property output (output/write_c_prop_data.cu:3-84): appends one row of the
+ * leaves' qx0, qy0 (MWDG2 also qx1x, qx1y, qy1x, qy1y) to
+ * <dirroot>qx0-c-prop.csv ... (dirroot and name joined without a separator,
+ * as the reference does) and records a cumulative row. */
+int mw_write_c_prop(mw_ctx* ctx);
+/* The saveint block of run_simulation.cu:656-705: the VTK file, raster maps
+ * and c-property rows that the .par asks for (vtk, raster_out, c_prop). */
+int mw_save_outputs(mw_ctx* ctx, int index);
+/* Cumulative output rows (output/write_cumulative_data.cpp:3-50), recorded on
+ * the device at every massint save with the .par keyword cumulative and on the
+ * host by mw_write_c_prop, in file order: rows of 7 doubles (time, next time
+ * step, leaf count, step start -> leaf list device clock ticks summed,
+ * device clock ticks since the first step, step, 0 c-property / 1 massint).
+ * *n_rows receives the row count; at most max_rows are copied. */
+int mw_cumulative_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows);
+/* Writes the rows as the reference's .cumu file; path NULL = dirroot/resroot.cumu. */
+int mw_write_cumulative_file(mw_ctx* ctx, const char* path);
 
 /* Drop-in for the reference executable: argv[argc-1] is the .par file
  * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */

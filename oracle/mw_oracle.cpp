@@ -32,6 +32,7 @@ std::vector<real>* real_array(Oracle<real>& o, const std::string& n) {
     if (n == hn[i]) return hier[i];
   if (n == "norm") return &o.norm;
   if (n == "stage") return &o.stage_rec;
+  if (n == "cumulative") return &o.cum_rec;
   if (n == "dt_cfl") return &o.dt_cfl;
   // det_<var>_<mode>_<family>, var in eta,qx,qy,z; mode 0,1x,1y; family a,b,g
   if (n.rfind("det_", 0) == 0) {

@@ -167,6 +167,7 @@ struct Params {
   int  xsz = 0, ysz = 0;
   real g = real(9.80665), time = 0, manning = 0;
   bool is_monai = false, is_oregon = false;
+  bool cumulative = false;                   // classes/PlottingParams.h:35
   real massint = real(9999.0);                // classes/SaveInterval.h: 9999/9999 without massint
   int massint_count0 = 9999;
   real dx = 0;                            // input/read_cell_size.cpp:3-45
@@ -240,7 +241,8 @@ struct Oracle {
   std::vector<PointSrc<real>> srcs;
   std::vector<int> stage_codes;
   int save_count = 9999;
-  std::vector<real> stage_rec;                // rows: time, depths, elevations
+  std::vector<real> stage_rec;                // rows: time, depths, elevations, x and y velocities
+  std::vector<real> cum_rec;                  // rows: time, next dt, leaf count (massint saves)
   void record_stage();
 
   int N = 0, D = 0, H = 0, mesh_dim = 0;

@@ -144,3 +144,127 @@ def test_stage_velocity_files(case_dir, solver):
         assert got.shape == (rows.shape[0], 1 + npt)
         assert np.allclose(got[:, 0], rows[:, 0], atol=6e-7)
         assert np.allclose(got[:, 1:], rows[:, 1 + col * npt:1 + (col + 1) * npt], atol=6e-7)
+
+
+def read_vtk(path):
+    lines = open(path).read().splitlines()
+    out = {}
+    i = lines.index("DATASET UNSTRUCTURED_GRID")
+    npts = int(lines[i + 1].split()[1])
+    out["points"] = np.array([[float(v) for v in l.split()] for l in lines[i + 2:i + 2 + npts]])
+    k = lines.index(next(l for l in lines if l.startswith("CELLS ")))
+    ncell = int(lines[k].split()[1])
+    out["cells"] = np.array([[int(v) for v in l.split()] for l in lines[k + 1:k + 1 + ncell]])
+    k = lines.index(f"CELL_TYPES {ncell}")
+    out["types"] = np.array([int(l) for l in lines[k + 1:k + 1 + ncell]])
+    for j, l in enumerate(lines):
+        if l.startswith("SCALARS "):
+            out[l.split()[1]] = np.array([float(v) for v in lines[j + 2:j + 2 + ncell]])
+    return npts, ncell, out
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_vtk_matches_oracle_leaves(case_dir, solver):
+    """VTK output (output/write_soln_vtk.cu:3-221): one pixel per leaf at its
+    Morton position, with each leaf's CFL time step (bit for bit against the
+    oracle's per-leaf dt of the last update) and h, qx, qy, z, eta of the
+    leaf to the printed digits."""
+    import mwdg2
+    d = os.path.join(str(case_dir), f"vtk_{solver}")
+    par = cases.random_case(d, L=5, seed=37, solver=solver, manning=0.02, t_end=1.0, xsz=29, ysz=23, startq=True)
+    with open(par, "a") as f:
+        f.write("vtk\ndirroot " + os.path.join(d, "out") + "\n")
+    o = oracle.Oracle(par, "f64")
+    with mwdg2.Simulation(par, "f64") as s:
+        for _ in range(5):
+            o.step()
+            s.step(1)
+        dtl = s.get("leaf_dt")
+        s.write_vtk(3)
+    n = int(o.scalar("length"))
+    act = o.get("assem.act")[:n].astype(np.int64)
+    lev = o.get("assem.lev")[:n].astype(np.int64)
+    dt_o = o.get("dt_cfl")[:n]
+    h = o.get("assem.h0")[:n]
+    z = o.get("assem.z0")[:n]
+    qx, qy = o.get("assem.qx0")[:n], o.get("assem.qy0")[:n]
+    L = int(o.scalar("L"))
+    o.close()
+    assert dtl.size == n and np.array_equal(dtl, dt_o)
+    assert np.any(dt_o != 1.0)
+    npts, ncell, v = read_vtk(os.path.join(d, "out", "res-3.vtk"))
+    assert npts == 4 * n and ncell == n
+    assert np.array_equal(v["cells"], np.column_stack([np.full(n, 4), 4 * np.arange(n)[:, None] + np.arange(4)]))
+    assert np.all(v["types"] == 8)
+    code = act - np.array([oracle.lvl_idx(int(l)) for l in lev])
+    cs = 2.0 ** (L - lev)
+    x = np.array([oracle.compact(int(c)) for c in code]) * cs
+    y = np.array([oracle.compact(int(c) >> 1) for c in code]) * cs
+    exp_pts = np.stack([np.column_stack([x, y]), np.column_stack([x, y + cs]), np.column_stack([x + cs, y]),
+                        np.column_stack([x + cs, y + cs])], axis=1).reshape(-1, 2)
+    assert np.allclose(v["points"][:, :2], exp_pts, atol=1e-6) and np.all(v["points"][:, 2] == 1.0)
+    hh = (h + z) - z
+    for name, ref in (("dt", dt_o), ("h", hh), ("qx", qx), ("qy", qy), ("z", z), ("eta", hh + z)):
+        assert np.allclose(v[name], ref, atol=6e-7, rtol=0), name
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+def test_c_prop_and_cumulative_files(case_dir, solver):
+    """c_prop and cumulative outputs of the drop-in runner
+    (output/write_c_prop_data.cu:3-84, write_cumulative_data.cpp:3-50,
+    run_simulation.cu:656-750): one row of leaf discharges per saveint save
+    (%.15f, string for string against the oracle's leaves), and the .cumu rows
+    of both blocks in file order (per step the saveint row, then the massint
+    row; the first step's massint row replaces its saveint row) with time, dt
+    and reduction from the oracle."""
+    import mwdg2
+    d = os.path.join(str(case_dir), f"cprop_{solver}")
+    par = cases.random_case(d, L=5, seed=41, solver=solver, manning=0.02, t_end=1.0, startq=True)
+    txt = open(par).read()
+    txt = txt.replace("massint       1.0", "massint       0.1").replace("saveint       1.0", "saveint       0.25")
+    out = os.path.join(d, "out") + "/"
+    with open(par, "w") as f:
+        f.write(txt + "c_prop\ncumulative\ndirroot " + out + "\n")
+    assert mwdg2.run_simulation(["mwdg2_run", par]) == 0
+    # the oracle, with the runner's saveint test after every step
+    names = ["qx0", "qy0"] + (["qx1x", "qx1y", "qy1x", "qy1y"] if solver == "mwdg2" else [])
+    o = oracle.Oracle(par, "f64")
+    rows = {k: [] for k in names}
+    cprop = []
+    count, step = 0, 0
+    while o.step():
+        step += 1
+        t = o.scalar("time")
+        if t >= 0.25 * count:
+            count += 1
+            n = int(o.scalar("length"))
+            for k in names:
+                rows[k].append("".join("%.15f," % v for v in o.get("assem." + k)[:n]))
+            cprop.append((t, o.scalar("dt"), n, step, 0))
+    cum = o.get("cumulative").reshape(-1, 3)
+    xsz, ysz = int(o.scalar("xsz")), int(o.scalar("ysz"))
+    o.close()
+    assert len(rows["qx0"]) >= 5
+    for k in names:
+        got = open(out + k + "-c-prop.csv").read().splitlines()
+        assert got == rows[k], k
+    # massint rows: the oracle records them at the same saves as the stage output
+    mass = [(r[0], r[1], int(r[2]), None, 1) for r in cum]
+    assert len(mass) >= 10
+    # the first step writes both rows; the massint one is kept
+    cprop = [r for r in cprop if r[3] != 1]
+    got = np.loadtxt(os.path.join(out, "res.cumu"), delimiter=",", skiprows=1)
+    assert open(os.path.join(out, "res.cumu")).readline().strip() == \
+        "simtime,runtime_mra,runtime_solver,runtime_total,dt,reduction"
+    assert got.shape == (len(mass) + len(cprop), 6)
+    exp = sorted([(r[0], 1, r) for r in cprop] + [(r[0], 2, r) for r in mass], key=lambda e: (e[0], e[1]))
+    for g, (_, _, r) in zip(got, exp):
+        assert abs(g[0] - r[0]) < 6e-7 and abs(g[4] - r[1]) < 6e-7
+        assert abs(g[5] - (100.0 - 100.0 * r[2] / (xsz * ysz)) / 100.0) < 6e-7
+    # runtimes: device clock, mra part <= total, increasing along each block's
+    # rows (a saveint row is timed at the host's save, after its step's
+    # massint row)
+    assert np.all(got[:, 1] >= 0) and np.all(got[:, 3] >= got[:, 1] - 1e-6)
+    kind = np.array([e[1] for e in exp])
+    for k in (1, 2):
+        assert np.all(np.diff(got[kind == k, 3]) >= -1e-6)
