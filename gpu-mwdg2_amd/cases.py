@@ -83,6 +83,20 @@ def _par(solver: str, L: int, eps: float, t_end: float, manning: float, extra=()
     return ent
 
 
+def synthetic(d: str, test_case: int, L: int = 6, solver: str = "mwdg2", eps: float = 1e-3, t_end: float = 5.0,
+              extra=()) -> str:
+    """A built-in synthetic case (the reference's test_case 1-23 but 20; no
+    input rasters: the domain, bed and initial depth come from the case
+    number)."""
+    os.makedirs(d, exist_ok=True)
+    ent = [("test_case", test_case), solver, ("max_ref_lvl", L), ("epsilon", repr(eps)), ("wall_height", "0"),
+           ("initial_tstep", "1"), ("sim_time", repr(t_end)), ("massint", repr(t_end)), ("saveint", repr(t_end))]
+    ent.extend(extra)
+    par = os.path.join(d, "case.par")
+    write_par(par, ent)
+    return par
+
+
 def circular_dam_break(d: str, L: int = 8, eps: float = 1e-3, t_end: float = 3.5,
                        solver: str = "mwdg2", cfl: float | None = 0.33) -> str:
     """Circular dam break on a flat, frictionless 50 m square with closed walls."""

@@ -296,6 +296,11 @@ struct Ctx : CtxBase {
     P.limitslopes = hc.limitslopes ? 1 : 0;
     P.monai = hc.monai ? 1 : 0;
     P.oregon = hc.oregon ? 1 : 0;
+    P.test_case = hc.test_case;
+    {
+      const int tc = hc.test_case;
+      P.flow = (tc >= 1 && tc <= 18) ? ((tc % 2 == 1) ? FLOW_EW : FLOW_NS) : FLOW_NONE;
+    }
     P.massint = hc.massint;
     for (int k = 0; k < 4; ++k) { P.btype[k] = hc.bdy[k].type; P.bstart[k] = hc.bdy[k].start; P.bend[k] = hc.bdy[k].end; }
   }

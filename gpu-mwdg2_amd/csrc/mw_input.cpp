@@ -175,13 +175,136 @@ std::string par_str(const char* file, const char* kw) {
   return std::string();
 }
 
+// Built-in synthetic cases. Domain, friction and the 1D depths per case
+// (classes/SimulationParams.cpp:15-147, classes/Depths1D.h:10-40).
+template <typename real>
+struct Synthetic {
+  real xmin, xmax, ymin, ymax, manning, hl, hr;
+};
+
+template <typename real>
+Synthetic<real> synthetic_params(int tc) {
+  Synthetic<real> s{real(0.0), real(50.0), real(0.0), real(50.0), real(0.0), real(0.0), real(0.0)};
+  if (tc == 9 || tc == 10 || (tc >= 11 && tc <= 14)) s.manning = real(0.02);
+  if (tc == 15 || tc == 16) { s.xmax = s.ymax = real(38.0); s.manning = real(0.0125); }
+  if (tc == 17 || tc == 18) { s.xmin = s.ymin = real(-50.0); s.xmax = s.ymax = real(50.0); }
+  if (tc == 19) { s.xmin = real(10.0); s.xmax = real(70.0); s.ymin = real(-10.0); s.ymax = real(50.0); }
+  if (tc == 21 || tc == 22) { s.xmax = s.ymax = real(75.0); }
+  if (tc == 23) { s.xmin = s.ymin = real(-20.0); s.xmax = s.ymax = real(20.0); }
+  if (tc == 1 || tc == 2) { s.hl = real(6.0); s.hr = real(6.0); }
+  else if (tc == 3 || tc == 4) { s.hl = real(2.0); s.hr = real(2.0); }
+  else if (tc == 5 || tc == 6 || tc == 11 || tc == 12) { s.hl = real(6.0); s.hr = real(2.0); }
+  else if ((tc >= 7 && tc <= 10) || tc == 13 || tc == 14) { s.hl = real(6.0); s.hr = real(0.0); }
+  else if (tc == 17) { s.hl = real(2.0); s.hr = real(0.0); }
+  return s;
+}
+
+// Bed elevation at a node (synthetic/topo*.cuh).
+template <typename real>
+real synthetic_bed(real x, real y, int tc) {
+  auto c_prop_1d = [](real a) {                               // topo_c_prop_1D.cuh
+    real z = real(0.0);
+    if (a >= real(22) && a < real(25)) z = real(0.05) * a - real(1.1);
+    else if (a >= real(25) && a <= real(28)) z = real(-0.05) * a + real(1.4);
+    else if (a > real(8) && a < real(12)) z = real(0.2) - real(0.05) * (a - real(10)) * (a - real(10));
+    else if (a > real(39) && a < real(46.5)) z = real(0.3);
+    return real(10.0) * z;
+  };
+  auto triangle = [](real a) {                                // topo_triangle.cuh
+    const real slope = real(0.4) / real(3.0);
+    if (a > real(25.5) && a <= real(28.5)) return (a - real(25.5)) * slope;
+    if (a > real(28.5) && a <= real(31.5)) return real(0.4) - (a - real(28.5)) * slope;
+    return real(0.0);
+  };
+  auto dist = [](real x, real y, real x0, real y0) { return std::sqrt((x - x0) * (x - x0) + (y - y0) * (y - y0)); };
+  switch (tc) {
+    case 1: case 3: case 11: case 13: return c_prop_1d(x);
+    case 2: case 4: case 12: case 14: return c_prop_1d(y);
+    case 15: return triangle(x);
+    case 16: return triangle(y);
+    case 17: return real(0.01) * x * x;                       // topo_parabolic_bowl.cuh
+    case 18: return real(0.01) * y * y;
+    case 19: case 20: {                                       // topo_three_cones.cuh
+      const real z1 = (real(8) - dist(x, y, real(30), real(6))) / real(8);
+      const real z2 = (real(8) - dist(x, y, real(30), real(24))) / real(8);
+      const real z3 = real(0.3) * (real(10) - dist(x, y, real(47.5), real(15)));
+      return std::fmax(std::fmax(std::fmax(z1, z2), z3), real(0.0));
+    }
+    case 21: {                                                // topo_diff.cuh
+      real z = real(0);
+      z = std::fmax(z, real(1.0) - real(0.2) * dist(x, y, real(20.0), real(15.0)));
+      z = std::fmax(z, real(2.0) - real(0.5) * dist(x, y, real(40.0), real(15.0)));
+      z = std::fmax(z, real(3.0) - real(0.3) * dist(x, y, real(60.0), real(15.0)));
+      return z;
+    }
+    case 22: {                                                // topo_non_diff.cuh
+      const bool yb = real(11.0) <= y && y <= real(19.0);
+      if (real(16.0) <= x && x <= real(24.0) && yb) return real(0.86);
+      if (real(36.0) <= x && x <= real(44.0) && yb) return real(1.78);
+      if (real(56.0) <= x && x <= real(64.0) && yb) return real(2.30);
+      return real(0);
+    }
+    default: return real(0.0);
+  }
+}
+
+// Depth at a node (synthetic/h_init*.cuh). The c-property depth is the
+// surface level minus the bed, negative above it, as the reference returns.
+template <typename real>
+real synthetic_depth(real x, real y, real z, const Synthetic<real>& s, int tc) {
+  auto overtop = [&](real a) { return (a <= real(25)) ? std::fmax(real(0.0), s.hl - z) : std::fmax(real(0.0), s.hr - z); };
+  switch (tc) {
+    case 1: case 2: case 3: case 4: return s.hl - z;
+    case 5: case 7: case 9: case 11: case 13: return overtop(x);
+    case 6: case 8: case 10: case 12: case 14: return overtop(y);
+    case 15: return (x < real(15.5)) ? real(0.75) : real(0.0);
+    case 16: return (y < real(15.5)) ? real(0.75) : real(0.0);
+    case 17: return std::fmax(real(0.0), (real(0.2) * x + real(10.0)) - z);
+    case 18: return std::fmax(real(0.0), (real(0.2) * y + real(10.0)) - z);
+    case 19: return real(1.0) - z;
+    case 20: return (x < real(16.0)) ? std::fmax(real(1.875) - z, real(0.0)) : real(0.0);
+    case 21: return real(1.95) - z;
+    case 22: return real(1.78) - z;
+    case 23: return (std::sqrt(x * x + y * y) < real(2.5)) ? real(2.5) : real(0.5);
+    default: return real(0.0);
+  }
+}
+
+// 4-node averages and (MWDG2) slopes of nodal values on the
+// (md + 1) x (md + 1) node grid (mra/modal_projections.cu:28-88).
+template <typename real>
+void project_nodes(HostCase<real>& c, int md, const std::vector<real> (&node)[4]) {
+  const size_t N = (size_t)md * md;
+  const int nd = md + 1;
+  for (auto& a : c.modal) a.assign(N, real(0));
+  const real s3 = std::sqrt(real(3.0));
+  for (size_t idx = 0; idx < N; ++idx) {
+    const size_t x = idx % md, y = idx / md;
+    const size_t SW = y * nd + x, SE = SW + 1, NW = (y + 1) * nd + x, NE = NW + 1;
+    for (int v = 0; v < 4; ++v) {
+      const std::vector<real>& a = node[v];
+      c.modal[3 * v][idx] = real(0.25) * (a[NW] + a[NE] + a[SE] + a[SW]);
+      if (c.solver != 1) {
+        c.modal[3 * v + 1][idx] = (a[NE] - a[NW] + a[SE] - a[SW]) / (real(4.0) * s3);
+        c.modal[3 * v + 2][idx] = (a[NE] - a[SE] + a[NW] - a[SW]) / (real(4.0) * s3);
+      }
+    }
+  }
+  if (c.solver == 1)
+    for (int v = 0; v < 12; ++v)
+      if (v % 3 != 0) c.modal[v].clear();
+}
+
 template <typename real>
 void read_case(const char* par, int argc, const char* const* argv, HostCase<real>& c) {
   {
     File probe(par);
   }
-  if (par_int(par, "test_case") != 0)                       // input/read_test_case.cpp:3-17
-    throw InputError("only raster-driven cases (test_case 0) are supported");
+  c.test_case = par_int(par, "test_case");                  // input/read_test_case.cpp:3-17
+  if (c.test_case < 0 || c.test_case > 23) throw InputError("test_case must be in [0, 23]");
+  if (c.test_case == 20)
+    throw InputError("test_case 20 (three cones dam break, a 70 x 30 domain on a square mesh) needs non-square "
+                     "cells, which are not supported");
   c.L = par_int(par, "max_ref_lvl");
   if (c.L < 2 || c.L > 14) throw InputError("max_ref_lvl must be in [2, 14]");
   c.initial_tstep = par_real<real>(par, "initial_tstep");
@@ -192,22 +315,32 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
   else throw InputError("invalid adaptive solver type: the .par needs a line 'hwfv1' or 'mwdg2'");
   c.limitslopes = par_has(par, "limitslopes");
   if (c.limitslopes) c.tol_Krivo = par_real<real>(par, "tol_Krivo");
-  c.refine_wall = par_has(par, "refine_wall");
+  c.refine_wall = c.test_case == 0 && par_has(par, "refine_wall");   // mra/preflag_details.cu:77
   if (c.refine_wall) c.ref_thickness = par_int(par, "ref_thickness");
   c.startq2d = par_has(par, "startq2d");
   if (par_has(par, "cfl")) c.CFL = par_real<real>(par, "cfl");    // extension keyword
 
-  const std::string dem = beside(par, par_str(par, "DEMfile"));
-  if (dem.empty()) throw InputError("the .par names no DEMfile");
-  c.xsz = par_int(dem.c_str(), "ncols");
-  c.ysz = par_int(dem.c_str(), "nrows");
-  c.xmin = par_real<real>(dem.c_str(), "xllcorner");
-  c.ymin = par_real<real>(dem.c_str(), "yllcorner");
-  c.dx = par_real<real>(dem.c_str(), "cellsize");
+  const int md = 1 << c.L;
+  const std::string dem = c.test_case == 0 ? beside(par, par_str(par, "DEMfile")) : std::string();
+  const Synthetic<real> syn = synthetic_params<real>(c.test_case);
+  if (c.test_case == 0) {
+    if (dem.empty()) throw InputError("the .par names no DEMfile");
+    c.xsz = par_int(dem.c_str(), "ncols");
+    c.ysz = par_int(dem.c_str(), "nrows");
+    c.xmin = par_real<real>(dem.c_str(), "xllcorner");
+    c.ymin = par_real<real>(dem.c_str(), "yllcorner");
+    c.dx = par_real<real>(dem.c_str(), "cellsize");
+    c.manning = par_real<real>(par, "fpfric");
+    c.monai = par_has(par, "monai");
+    c.oregon = par_has(par, "oregon-seaside");
+  } else {                                                   // run_simulation.cu:35-36
+    c.xsz = c.ysz = md;
+    c.xmin = syn.xmin;
+    c.ymin = syn.ymin;
+    c.dx = (syn.xmax - syn.xmin) / real(md);
+    c.manning = syn.manning;
+  }
   c.sim_time = par_real<real>(par, "sim_time");
-  c.manning = par_real<real>(par, "fpfric");
-  c.monai = par_has(par, "monai");
-  c.oregon = par_has(par, "oregon-seaside");
   if (par_has(par, "massint")) {
     c.massint = par_real<real>(par, "massint");
     c.massint_count0 = 0;
@@ -239,12 +372,11 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
       else throw InputError("invalid -solver value");
     }
   }
-  const int md = 1 << c.L;
   if (c.xsz < 1 || c.ysz < 1 || c.xsz > md || c.ysz > md) throw InputError("DEM size does not fit the mesh");
   if (!(c.dx > real(0))) throw InputError("DEM cellsize must be positive");
 
   // boundaries (classes/Boundary.h:237-369)
-  const std::string bci_name = par_str(par, "bcifile");
+  const std::string bci_name = c.test_case == 0 ? par_str(par, "bcifile") : std::string();   // Boundary.h:93
   const std::string bci = beside(par, bci_name);
   const char dchar[4] = {'N', 'E', 'S', 'W'};
   if (!bci_name.empty()) {
@@ -329,8 +461,22 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
     }
   }
 
-  // initial modal state (mra/get_modal_values.cu:29-52)
+  // initial modal state (mra/get_modal_values.cu:14-52)
   const size_t N = (size_t)md * md;
+  if (c.test_case != 0) {                                    // mra/init_nodal_values.cu:45-81
+    const int nd = md + 1;
+    std::vector<real> node[4];
+    for (auto& a : node) a.assign((size_t)nd * nd, real(0));
+    for (int yi = 0; yi < nd; ++yi)
+      for (int xi = 0; xi < nd; ++xi) {
+        const real x = c.xmin + c.dx * real(xi), y = c.ymin + c.dx * real(yi);
+        const size_t i = (size_t)yi * nd + xi;
+        node[3][i] = synthetic_bed<real>(x, y, c.test_case);
+        node[0][i] = synthetic_depth<real>(x, y, node[3][i], syn, c.test_case);
+      }
+    project_nodes<real>(c, md, node);
+    return;
+  }
   std::vector<real> hr(N, real(0)), qxr(N, real(0)), qyr(N, real(0)), zr(N, c.wall_height);
   const std::string start = beside(par, par_str(par, "startfile"));
   if (!start.empty()) read_raster<real>(start, hr, md, c.wall_height);
@@ -361,18 +507,7 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
       const size_t ii = (size_t)(nd - 1 - y) * nd + x, mi = (size_t)(md - 1 - j) * md + i;
       for (int v = 0; v < 4; ++v) node[v][ii] = (*src[v])[mi];
     }
-  for (auto& a : c.modal) a.assign(N, real(0));
-  const real s3 = std::sqrt(real(3.0));
-  for (size_t idx = 0; idx < N; ++idx) {
-    const size_t x = idx % md, y = idx / md;
-    const size_t SW = y * nd + x, SE = SW + 1, NW = (y + 1) * nd + x, NE = NW + 1;
-    for (int v = 0; v < 4; ++v) {
-      const std::vector<real>& a = node[v];
-      c.modal[3 * v][idx] = real(0.25) * (a[NW] + a[NE] + a[SE] + a[SW]);
-      c.modal[3 * v + 1][idx] = (a[NE] - a[NW] + a[SE] - a[SW]) / (real(4.0) * s3);
-      c.modal[3 * v + 2][idx] = (a[NE] - a[SE] + a[NW] - a[SW]) / (real(4.0) * s3);
-    }
-  }
+  project_nodes<real>(c, md, node);
 }
 
 template void read_case<float>(const char*, int, const char* const*, HostCase<float>&);

@@ -39,6 +39,8 @@ struct HostSource {                   // classes/PointSources.h:15-473
 
 template <typename real>
 struct HostCase {
+  // built-in synthetic case (input/read_test_case.cpp:3-17); 0 = raster files
+  int test_case = 0;
   // solver parameters (classes/SolverParams.cu:10-47)
   int L = 0, solver = 0;
   real epsilon = 0, wall_height = 0, initial_tstep = 0, CFL = 0, tol_Krivo = real(9999.0);

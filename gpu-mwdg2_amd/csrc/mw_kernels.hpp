@@ -55,7 +55,12 @@ struct Params {
   int oregon;                     // west HFIX/HVAR is the Oregon Seaside wavemaker (if not monai)
   real massint;                   // stage-output interval
   real* leaf_dt;                  // per-leaf CFL time step of the last update (VTK output), or null
+  int test_case;                  // 0: raster case; else the built-in synthetic case
+  int flow;                       // synthetic cases: FLOW_EW, FLOW_NS or FLOW_NONE
 };
+
+// Flow direction of the synthetic cases (neighbours/add_ghost_cells.cu:36-62).
+enum { FLOW_NONE = 0, FLOW_EW = 1, FLOW_NS = 2 };
 
 // Flow scale coefficients of every node of the hierarchy, eta form. Coefficient
 // v = 3 * var + mode: eta0 eta1x eta1y qx0 qx1x qx1y qy0 qy1x qy1y.

@@ -244,6 +244,7 @@ struct Oracle {
   std::vector<real> stage_rec;                // rows: time, depths, elevations, x and y velocities
   std::vector<real> cum_rec;                  // rows: time, next dt, leaf count (massint saves)
   void record_stage();
+  void load_synthetic(const char* par);
 
   int N = 0, D = 0, H = 0, mesh_dim = 0;
   // hierarchy (classes/ScaleCoefficients.h:47-60)

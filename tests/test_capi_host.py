@@ -79,7 +79,8 @@ def _par(tmp_path, name, edit=None, files=None, **kw):
 @pytest.mark.parametrize("name,edit,files,needle", [
     ("missing_par", None, None, "cannot open"),
     ("level", [("max_ref_lvl   4", "max_ref_lvl   15")], None, "max_ref_lvl"),
-    ("testcase", [("test_case     0", "test_case     3")], None, "test_case 0"),
+    ("testcase", [("test_case     0", "test_case     24")], None, "[0, 23]"),
+    ("testcase20", [("test_case     0", "test_case     20")], None, "non-square"),
     ("solver", [("mwdg2\n", "\n")], None, "solver"),
     ("dem", [("case.dem", "nothere.dem")], None, "cannot open"),
     ("bigdem", None, {"case.dem": "ncols 40\nnrows 4\nxllcorner 0\nyllcorner 0\ncellsize 1\nNODATA_value -9999\n"

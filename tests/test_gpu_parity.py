@@ -148,3 +148,35 @@ def test_oregon_wavemaker(case_dir, solver, prec):
         f.write("oregon-seaside\n")
     reps = parity_util.run_pair(par, prec, steps=30, t0=14.6)
     assert len(reps) == 30
+
+
+@pytest.mark.parametrize("tc,solver,prec", [
+    (1, "mwdg2", "f64"), (4, "mwdg2", "f64"), (19, "mwdg2", "f64"), (22, "mwdg2", "f64"),
+    (5, "mwdg2", "f64"), (8, "mwdg2", "f64"), (15, "mwdg2", "f64"), (16, "mwdg2", "f64"), (17, "mwdg2", "f64"),
+    (23, "mwdg2", "f64"), (11, "mwdg2", "f32"), (2, "hwfv1", "f64"), (21, "hwfv1", "f64"), (6, "hwfv1", "f64"),
+    (18, "hwfv1", "f32"),
+])
+def test_synthetic_cases(case_dir, tc, solver, prec):
+    """Built-in synthetic cases (test_case 1-23 but 20): nodal initial state,
+    open borders with the flow-direction slope rules and the case walls
+    (neighbours/add_ghost_cells.cu:36-432), against the oracle bit for bit."""
+    par = _mk(case_dir, f"syn{tc}_{solver}_{prec}", cases.synthetic, test_case=tc, L=6, solver=solver, t_end=30.0)
+    reps = parity_util.run_pair(par, prec, steps=25)
+    assert len(reps) == 25
+
+
+def test_c_property_on_gpu(case_dir):
+    """The c-property check of the reference's suite (cases/tests/test.py:105,
+    396) on the GPU: a lake at rest over the differentiable three-cone bed
+    stays at rest (every discharge coefficient below 1e-10, FP64) over 200
+    adaptive steps at max_ref_lvl 8."""
+    import mwdg2
+    par = _mk(case_dir, "cprop21", cases.synthetic, test_case=21, L=8, solver="mwdg2", t_end=100.0)
+    with mwdg2.Simulation(par, "f64") as s:
+        s.step(200)
+        c = s.counters()
+        leaves = s.get("leaves")
+        worst = max(float(np.abs(s.get("A." + f)[leaves]).max())
+                    for f in ("qx0", "qy0", "qx1x", "qx1y", "qy1x", "qy1y"))
+    assert c["steps"] == 200 and c["n_leaves"] < 4 ** 8
+    assert worst < 1e-10, worst

@@ -277,3 +277,70 @@ def test_oregon_ghost_closed_form(tmp_path):
             assert np.all(o.get("nb1.qx0")[:n][east] == 0.0)
     finally:
         o.close()
+
+
+def _c_prop_1d(a):
+    """Bed of the 1D c-property cases (synthetic/topo_c_prop_1D.cuh), from its
+    piecewise definition: two ramps meeting at a = 25, a parabolic bump centred
+    at a = 10 and a plateau on (39, 46.5), all scaled by 10."""
+    if 22 <= a < 25:
+        return 10 * (0.05 * a - 1.1)
+    if 25 <= a <= 28:
+        return 10 * (-0.05 * a + 1.4)
+    if 8 < a < 12:
+        return 10 * (0.2 - 0.05 * (a - 10) ** 2)
+    if 39 < a < 46.5:
+        return 3.0
+    return 0.0
+
+
+@pytest.mark.parametrize("tc,eta", [(1, 6.0), (2, 6.0), (19, 1.0), (21, 1.95), (22, 1.78)])
+def test_synthetic_initial_state(tmp_path, tc, eta):
+    """Built-in cases (synthetic/*.cuh, mra/modal_projections.cu:28-88): the
+    c-property cases start as a lake at rest, eta = h + z constant (its
+    slopes zero) and q = 0; cases 1 and 2 are the same bed along x and y,
+    and a cell's bed mean is the mean of the bed at its 4 corners."""
+    par = cases.synthetic(str(tmp_path / f"s{tc}"), tc, L=5)
+    o = oracle.Oracle(par, "f64")
+    try:
+        N = 1 << 10
+        fl = oracle.lvl_idx(5)
+        e0 = o.get("eta0")[fl:fl + N]
+        assert np.allclose(e0, eta, atol=1e-13, rtol=0)
+        for f in ("eta1x", "eta1y", "qx0", "qy0", "qx1x", "qy1y"):
+            assert np.allclose(o.get(f)[fl:fl + N], 0.0, atol=1e-13), f
+        z0 = o.get("z0")[fl:fl + N]
+        dx = 50.0 / 32
+        if tc in (1, 2):
+            for code in range(N):
+                x, y = oracle.compact(code), oracle.compact(code >> 1)
+                a = (x if tc == 1 else y)
+                exp = 0.5 * (_c_prop_1d(a * dx) + _c_prop_1d((a + 1) * dx))
+                assert abs(z0[code] - exp) < 1e-12, (code, z0[code], exp)
+            zs = o.get("z1y" if tc == 1 else "z1x")[fl:fl + N]
+            assert np.all(zs == 0.0)
+        assert np.isclose(float(o.scalar("dx")), {1: 50, 2: 50, 19: 60, 21: 75, 22: 75}[tc] / 32, rtol=0, atol=1e-15)
+    finally:
+        o.close()
+
+
+@pytest.mark.parametrize("solver", ["mwdg2", "hwfv1"])
+@pytest.mark.parametrize("tc", [1, 2, 3, 4, 19, 21, 22])
+def test_c_property_lake_at_rest(tmp_path, solver, tc):
+    """The reference's c-property suite (cases/tests/test.py:105, plotted
+    against 1e-10 at :396): a lake at rest over the 1D, three-cone,
+    differentiable and non-differentiable beds, wet and wet/dry, stays at rest
+    through adaptive steps (eps = 1e-3): every discharge coefficient stays
+    below 1e-10 in FP64."""
+    par = cases.synthetic(str(tmp_path / f"c{tc}{solver}"), tc, L=5, solver=solver, t_end=20.0)
+    o = oracle.Oracle(par, "f64")
+    try:
+        steps = o.steps(60)
+        assert steps >= 30
+        n = int(o.scalar("length"))
+        assert n < 1024                                    # the grid coarsened
+        names = ["qx0", "qy0"] + (["qx1x", "qx1y", "qy1x", "qy1y"] if solver == "mwdg2" else [])
+        worst = max(float(np.abs(o.get("assem." + f)[:n]).max()) for f in names)
+        assert worst < 1e-10, worst
+    finally:
+        o.close()
