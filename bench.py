@@ -78,7 +78,7 @@ def measured_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` (read + write) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, each counter
     scaled by its reading for a 1 GiB copy kernel; None if not recorded."""
-    path = os.path.join(ROOT, "profiles", "update_mra_kernels_urban_L12.md")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_urban_L12.txt")
     try:
         with open(path) as f:
             for line in f:

@@ -384,123 +384,77 @@ struct Ctx : CtxBase {
 
   void build_initial() {
     zorder_build();
-    // row-major modal values -> Z-ordered finest level
-    std::vector<real*> dm(12, nullptr);
-    for (int v = 0; v < 12; ++v)
-      if (!hc.modal[v].empty()) {
-        dm[v] = dalloc<real>(N);
-        MW_HIP(hipMemcpyAsync(dm[v], hc.modal[v].data(), sizeof(real) * N, hipMemcpyHostToDevice, stream));
-      } else {
-        dm[v] = nullptr;
+    // finest level of A and Zh, and each cell's depth for the maxima
+    real* h0 = dalloc<real>(N);
+    if (!hc.modal[0].empty()) {
+      // row-major modal values given through mw_set_initial -> Z order
+      // (zorder/sort_finest_scale_coeffs_z_order.cu, copy_finest_coefficients.cu)
+      std::vector<real*> dm(12, nullptr);
+      for (int v = 0; v < 12; ++v)
+        if (!hc.modal[v].empty()) {
+          dm[v] = dalloc<real>(N);
+          MW_HIP(hipMemcpyAsync(dm[v], hc.modal[v].data(), sizeof(real) * N, hipMemcpyHostToDevice, stream));
+        }
+      real** dtab = dalloc<real*>(12);
+      MW_HIP(hipMemcpyAsync(dtab, dm.data(), sizeof(real*) * 12, hipMemcpyHostToDevice, stream));
+      if (mwm)
+        hipLaunchKernelGGL((k_set_finest<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab,
+                           rev_z, hier[0], zh, L, N, h0);
+      else
+        hipLaunchKernelGGL((k_set_finest<real, false>), dim3(nblk(N)), dim3(WG), 0, stream,
+                           (const real* const*)dtab, rev_z, hier[0], zh, L, N, h0);
+      MW_HIP(hipGetLastError());
+      MW_HIP(hipStreamSynchronize(stream));
+      for (real* p : dm) if (p) (void)hipFree(p);
+      (void)hipFree(dtab);
+    } else {
+      // rasters (or a synthetic case) -> nodes -> modes, on the device
+      InitSrc<real> S{};
+      std::vector<real*> dr;
+      for (int v = 0; v < 4; ++v) {
+        S.r[v] = nullptr;
+        if (hc.raster[v].empty()) continue;
+        real* d = dalloc<real>(N);
+        MW_HIP(hipMemcpyAsync(d, hc.raster[v].data(), sizeof(real) * N, hipMemcpyHostToDevice, stream));
+        S.r[v] = d;
+        dr.push_back(d);
       }
-    real** dtab = dalloc<real*>(12);
-    MW_HIP(hipMemcpyAsync(dtab, dm.data(), sizeof(real*) * 12, hipMemcpyHostToDevice, stream));
-    if (mwm)
-      hipLaunchKernelGGL((k_set_finest<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab, rev_z,
-                         hier[0], zh, L, N);
-    else
-      hipLaunchKernelGGL((k_set_finest<real, false>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab,
-                         rev_z, hier[0], zh, L, N);
-    MW_HIP(hipGetLastError());
-    MW_HIP(hipStreamSynchronize(stream));
-    for (real* p : dm) if (p) (void)hipFree(p);
-    (void)hipFree(dtab);
+      if (hc.test_case == 0 && (!S.r[0] || !S.r[3])) throw InputError("no initial rasters (mw_set_initial not called)");
+      S.xsz = hc.xsz; S.ysz = hc.ysz; S.test_case = hc.test_case;
+      S.xmin = hc.xmin; S.ymin = hc.ymin; S.dx = hc.dx; S.hl = hc.syn_hl; S.hr = hc.syn_hr;
+      S.s3 = std::sqrt(real(3.0));
+      if (mwm)
+        hipLaunchKernelGGL((k_init_cells<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
+      else
+        hipLaunchKernelGGL((k_init_cells<real, false>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
+      MW_HIP(hipGetLastError());
+      MW_HIP(hipStreamSynchronize(stream));
+      for (real* p : dr) (void)hipFree(p);
+    }
 
     // preflags from gauges, boundaries, sources, walls (mra/preflag_details.cu:15-92)
-    std::vector<uint8_t> pf(D + 4, 0);
-    const int st0 = lvl_idx(L - 1);
-    for (int c : hc.stage_codes) pf[st0 + c / 4] = 1;
-    for (int k = 0; k < 4; ++k) for (int c : hc.bdy[k].codes) pf[st0 + c / 4] = 1;
-    for (auto& s : hc.srcs) pf[st0 + s.code / 4] = 1;
-    if (hc.refine_wall) {                                   // mra/refine_high_wall.cu:3-57
-      for (int j = 0; j < hc.ref_thickness; ++j)
-        for (int i = 0; i < hc.xsz; ++i) { const int y = hc.ysz - 1 - j; if (y >= 0) pf[st0 + morton(i, y) / 4] = 1; }
-      for (int j = 0; j < hc.ref_thickness; ++j)
-        for (int i = 0; i < hc.ysz; ++i) { const int x = hc.xsz - 1 - j; if (x >= 0) pf[st0 + morton(x, i) / 4] = 1; }
-    }
-    MW_HIP(hipMemcpy(preflag, pf.data(), D + 4, hipMemcpyHostToDevice));
-
-    // maxima of the initial assembled solution, including the point-source
-    // insertion into it (mra/insert_point_srcs.cu:3-29, get_max_scale_coeffs.cu)
-    real mx[4] = {0, 0, 0, 0};
     {
-      const std::vector<real>& h0 = hc.modal[0];
-      const std::vector<real>& qx0 = hc.modal[3];
-      const std::vector<real>& z0 = hc.modal[9];
-      std::map<int, real> hmod;   // Z index -> modified h0
-      for (int s = 0; s < n_src; ++s) {
-        const int code = hc.srcs[s].code;
-        const int rm = compact(code >> 1) * md + compact(code);
-        const real cur_h = hmod.count(code) ? hmod[code] : h0[rm];
-        real val = hc.srcs[s].value;
-        int row = 0;
-        if (hc.srcs[s].type == HVAR || hc.srcs[s].type == QVAR)
-          advance_series_host<real>(real(0.001), row, hc.srcs[s].series.t, hc.srcs[s].series.v, val);
-        if (hc.srcs[s].type == HFIX || hc.srcs[s].type == HVAR) {
-          const int rs = compact(s >> 1) * md + compact(s);   // z0 at Z index s (quirk)
-          hmod[code] = val - z0[rs];
-        } else {
-          hmod[code] = cur_h + val * real(0.001) / hc.dx;
-        }
-      }
-      std::vector<int> rm_of_mod;
-      for (size_t i = 0; i < (size_t)N; ++i) {
-        const real h = h0[i];
-        const real e = rabs<real>(h + z0[i]);
-        mx[0] = e > mx[0] ? e : mx[0];
-        const real ah = rabs<real>(h);
-        mx[1] = ah > mx[1] ? ah : mx[1];
-        const real aq = rabs<real>(qx0[i]);
-        mx[2] = aq > mx[2] ? aq : mx[2];
-        const real az = rabs<real>(z0[i]);
-        mx[3] = az > mx[3] ? az : mx[3];
-      }
-      // the maxima must be those of the modified values: rescan if a source
-      // lowered a cell that held a maximum
-      if (!hmod.empty()) {
-        std::vector<char> is_mod(N, 0);
-        for (auto& kv : hmod) is_mod[compact(kv.first >> 1) * md + compact(kv.first)] = 1;
-        real m0 = 0, m1 = 0;
-        for (size_t i = 0; i < (size_t)N; ++i) {
-          real h = h0[i];
-          if (is_mod[i]) {
-            const int code = morton((int)(i % md), (int)(i / md));
-            h = hmod[code];
-          }
-          const real e = rabs<real>(h + z0[i]);
-          m0 = e > m0 ? e : m0;
-          const real ah = rabs<real>(h);
-          m1 = ah > m1 ? ah : m1;
-        }
-        mx[0] = m0;
-        mx[1] = m1;
-      }
+      std::vector<int> codes(hc.stage_codes.begin(), hc.stage_codes.end());
+      for (int k = 0; k < 4; ++k) codes.insert(codes.end(), hc.bdy[k].codes.begin(), hc.bdy[k].codes.end());
+      for (auto& sr : hc.srcs) codes.push_back(sr.code);
+      const int thick = hc.refine_wall ? hc.ref_thickness : 0;
+      const long long work = (long long)codes.size() + (long long)thick * (hc.xsz + hc.ysz);
+      int* dcodes = dalloc<int>(codes.size() + 1);
+      if (!codes.empty())
+        MW_HIP(hipMemcpyAsync(dcodes, codes.data(), sizeof(int) * codes.size(), hipMemcpyHostToDevice, stream));
+      MW_HIP(hipMemsetAsync(preflag, 0, D + 4, stream));
+      if (work > 0)
+        hipLaunchKernelGGL(k_preflag_init, dim3(nblk(work)), dim3(WG), 0, stream, preflag, dcodes, (int)codes.size(), L,
+                           thick, hc.xsz, hc.ysz);
+      MW_HIP(hipGetLastError());
+      MW_HIP(hipStreamSynchronize(stream));
+      (void)hipFree(dcodes);
     }
-    real mxs[5];
-    mxs[0] = rmax<real>(real(1.0), mx[0]);
-    mxs[1] = rmax<real>(real(1.0), mx[1]);
-    mxs[2] = rmax<real>(real(1.0), mx[2]);
-    mxs[3] = mxs[2];
-    mxs[4] = rmax<real>(real(1.0), mx[3]);
 
-    // z hierarchy and topography preflags (mra/preflag_topo.cu:3-43)
-    for (int level = L - 1; level >= 0; --level) {
-      if (mwm)
-        hipLaunchKernelGGL((k_z_encode<real, true>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
-                           level, L, hc.epsilon, mxs[4]);
-      else
-        hipLaunchKernelGGL((k_z_encode<real, false>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
-                           level, L, hc.epsilon, mxs[4]);
-    }
-    MW_HIP(hipGetLastError());
-    // all parents significant before the first encode (mra/init_sig_details.cu)
-    cur = 0;
-    MW_HIP(hipMemsetAsync(sig[cur], 1, D + 4, stream));
     DevState<real> s{};
     s.time = real(0);
     s.dt = real(0.001);                                      // run_simulation.cu:37
     s.sim_time = hc.sim_time;
-    for (int k = 0; k < 5; ++k) s.mx[k] = mxs[k];
     s.n_leaves = N;
     s.done = 0;
     s.first = 1;
@@ -510,6 +464,38 @@ struct Ctx : CtxBase {
     s.save_count = hc.massint_count0;
     s.n_records = 0;
     MW_HIP(hipMemcpyAsync(st, &s, sizeof s, hipMemcpyHostToDevice, stream));
+
+    // maxima of the initial assembled solution with the point sources
+    // inserted (mra/insert_point_srcs.cu:3-29, get_max_scale_coeffs.cu)
+    {
+      const int nparts = std::min(nblk(N), 1024);
+      real* part = dalloc<real>((size_t)4 * nparts);
+      if (mwm)
+        hipLaunchKernelGGL((k_init_maxima<real, true>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
+                           sources(), real(0.001), hc.dx, part);
+      else
+        hipLaunchKernelGGL((k_init_maxima<real, false>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
+                           sources(), real(0.001), hc.dx, part);
+      hipLaunchKernelGGL((k_init_maxima_final<real>), dim3(1), dim3(64), 0, stream, st, part, nparts);
+      MW_HIP(hipGetLastError());
+      MW_HIP(hipStreamSynchronize(stream));
+      (void)hipFree(part);
+    }
+    (void)hipFree(h0);
+
+    // z hierarchy and topography preflags (mra/preflag_topo.cu:3-43)
+    for (int level = L - 1; level >= 0; --level) {
+      if (mwm)
+        hipLaunchKernelGGL((k_z_encode<real, true>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+                           level, L, hc.epsilon, (const DevState<real>*)st);
+      else
+        hipLaunchKernelGGL((k_z_encode<real, false>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+                           level, L, hc.epsilon, (const DevState<real>*)st);
+    }
+    MW_HIP(hipGetLastError());
+    // all parents significant before the first encode (mra/init_sig_details.cu)
+    cur = 0;
+    MW_HIP(hipMemsetAsync(sig[cur], 1, D + 4, stream));
     MW_HIP(hipStreamSynchronize(stream));
     main_h = 0;
     issued_first = false;

@@ -199,102 +199,6 @@ Synthetic<real> synthetic_params(int tc) {
   return s;
 }
 
-// Bed elevation at a node (synthetic/topo*.cuh).
-template <typename real>
-real synthetic_bed(real x, real y, int tc) {
-  auto c_prop_1d = [](real a) {                               // topo_c_prop_1D.cuh
-    real z = real(0.0);
-    if (a >= real(22) && a < real(25)) z = real(0.05) * a - real(1.1);
-    else if (a >= real(25) && a <= real(28)) z = real(-0.05) * a + real(1.4);
-    else if (a > real(8) && a < real(12)) z = real(0.2) - real(0.05) * (a - real(10)) * (a - real(10));
-    else if (a > real(39) && a < real(46.5)) z = real(0.3);
-    return real(10.0) * z;
-  };
-  auto triangle = [](real a) {                                // topo_triangle.cuh
-    const real slope = real(0.4) / real(3.0);
-    if (a > real(25.5) && a <= real(28.5)) return (a - real(25.5)) * slope;
-    if (a > real(28.5) && a <= real(31.5)) return real(0.4) - (a - real(28.5)) * slope;
-    return real(0.0);
-  };
-  auto dist = [](real x, real y, real x0, real y0) { return std::sqrt((x - x0) * (x - x0) + (y - y0) * (y - y0)); };
-  switch (tc) {
-    case 1: case 3: case 11: case 13: return c_prop_1d(x);
-    case 2: case 4: case 12: case 14: return c_prop_1d(y);
-    case 15: return triangle(x);
-    case 16: return triangle(y);
-    case 17: return real(0.01) * x * x;                       // topo_parabolic_bowl.cuh
-    case 18: return real(0.01) * y * y;
-    case 19: case 20: {                                       // topo_three_cones.cuh
-      const real z1 = (real(8) - dist(x, y, real(30), real(6))) / real(8);
-      const real z2 = (real(8) - dist(x, y, real(30), real(24))) / real(8);
-      const real z3 = real(0.3) * (real(10) - dist(x, y, real(47.5), real(15)));
-      return std::fmax(std::fmax(std::fmax(z1, z2), z3), real(0.0));
-    }
-    case 21: {                                                // topo_diff.cuh
-      real z = real(0);
-      z = std::fmax(z, real(1.0) - real(0.2) * dist(x, y, real(20.0), real(15.0)));
-      z = std::fmax(z, real(2.0) - real(0.5) * dist(x, y, real(40.0), real(15.0)));
-      z = std::fmax(z, real(3.0) - real(0.3) * dist(x, y, real(60.0), real(15.0)));
-      return z;
-    }
-    case 22: {                                                // topo_non_diff.cuh
-      const bool yb = real(11.0) <= y && y <= real(19.0);
-      if (real(16.0) <= x && x <= real(24.0) && yb) return real(0.86);
-      if (real(36.0) <= x && x <= real(44.0) && yb) return real(1.78);
-      if (real(56.0) <= x && x <= real(64.0) && yb) return real(2.30);
-      return real(0);
-    }
-    default: return real(0.0);
-  }
-}
-
-// Depth at a node (synthetic/h_init*.cuh). The c-property depth is the
-// surface level minus the bed, negative above it, as the reference returns.
-template <typename real>
-real synthetic_depth(real x, real y, real z, const Synthetic<real>& s, int tc) {
-  auto overtop = [&](real a) { return (a <= real(25)) ? std::fmax(real(0.0), s.hl - z) : std::fmax(real(0.0), s.hr - z); };
-  switch (tc) {
-    case 1: case 2: case 3: case 4: return s.hl - z;
-    case 5: case 7: case 9: case 11: case 13: return overtop(x);
-    case 6: case 8: case 10: case 12: case 14: return overtop(y);
-    case 15: return (x < real(15.5)) ? real(0.75) : real(0.0);
-    case 16: return (y < real(15.5)) ? real(0.75) : real(0.0);
-    case 17: return std::fmax(real(0.0), (real(0.2) * x + real(10.0)) - z);
-    case 18: return std::fmax(real(0.0), (real(0.2) * y + real(10.0)) - z);
-    case 19: return real(1.0) - z;
-    case 20: return (x < real(16.0)) ? std::fmax(real(1.875) - z, real(0.0)) : real(0.0);
-    case 21: return real(1.95) - z;
-    case 22: return real(1.78) - z;
-    case 23: return (std::sqrt(x * x + y * y) < real(2.5)) ? real(2.5) : real(0.5);
-    default: return real(0.0);
-  }
-}
-
-// 4-node averages and (MWDG2) slopes of nodal values on the
-// (md + 1) x (md + 1) node grid (mra/modal_projections.cu:28-88).
-template <typename real>
-void project_nodes(HostCase<real>& c, int md, const std::vector<real> (&node)[4]) {
-  const size_t N = (size_t)md * md;
-  const int nd = md + 1;
-  for (auto& a : c.modal) a.assign(N, real(0));
-  const real s3 = std::sqrt(real(3.0));
-  for (size_t idx = 0; idx < N; ++idx) {
-    const size_t x = idx % md, y = idx / md;
-    const size_t SW = y * nd + x, SE = SW + 1, NW = (y + 1) * nd + x, NE = NW + 1;
-    for (int v = 0; v < 4; ++v) {
-      const std::vector<real>& a = node[v];
-      c.modal[3 * v][idx] = real(0.25) * (a[NW] + a[NE] + a[SE] + a[SW]);
-      if (c.solver != 1) {
-        c.modal[3 * v + 1][idx] = (a[NE] - a[NW] + a[SE] - a[SW]) / (real(4.0) * s3);
-        c.modal[3 * v + 2][idx] = (a[NE] - a[SE] + a[NW] - a[SW]) / (real(4.0) * s3);
-      }
-    }
-  }
-  if (c.solver == 1)
-    for (int v = 0; v < 12; ++v)
-      if (v % 3 != 0) c.modal[v].clear();
-}
-
 template <typename real>
 void read_case(const char* par, int argc, const char* const* argv, HostCase<real>& c) {
   {
@@ -463,51 +367,26 @@ void read_case(const char* par, int argc, const char* const* argv, HostCase<real
 
   // initial modal state (mra/get_modal_values.cu:14-52)
   const size_t N = (size_t)md * md;
-  if (c.test_case != 0) {                                    // mra/init_nodal_values.cu:45-81
-    const int nd = md + 1;
-    std::vector<real> node[4];
-    for (auto& a : node) a.assign((size_t)nd * nd, real(0));
-    for (int yi = 0; yi < nd; ++yi)
-      for (int xi = 0; xi < nd; ++xi) {
-        const real x = c.xmin + c.dx * real(xi), y = c.ymin + c.dx * real(yi);
-        const size_t i = (size_t)yi * nd + xi;
-        node[3][i] = synthetic_bed<real>(x, y, c.test_case);
-        node[0][i] = synthetic_depth<real>(x, y, node[3][i], syn, c.test_case);
-      }
-    project_nodes<real>(c, md, node);
-    return;
-  }
-  std::vector<real> hr(N, real(0)), qxr(N, real(0)), qyr(N, real(0)), zr(N, c.wall_height);
-  const std::string start = beside(par, par_str(par, "startfile"));
-  if (!start.empty()) read_raster<real>(start, hr, md, c.wall_height);
-  read_raster<real>(dem, zr, md, c.wall_height);
-  if (c.startq2d && !start.empty()) {
-    read_raster<real>(start + ".Qx", qxr, md, c.wall_height);
-    read_raster<real>(start + ".Qy", qyr, md, c.wall_height);
-  }
   for (auto& a : c.modal) a.clear();
-  if (c.solver == 1) {                                          // input/read_and_project_modes_fv1.cu:39-53
-    c.modal[0] = std::move(hr);
-    c.modal[3] = std::move(qxr);
-    c.modal[6] = std::move(qyr);
-    c.modal[9] = std::move(zr);
+  for (auto& a : c.raster) a.clear();
+  if (c.test_case != 0) {                   // nodal values and projection on the device
+    c.syn_hl = syn.hl;
+    c.syn_hr = syn.hr;
     return;
   }
-  // nodes from cells with the reference's flips and duplicated last row/column
-  // (input/read_and_project_modes_dg2.cu:51-80), then 4-node averages and
-  // slopes (mra/modal_projections.cu:71-86)
-  const int nd = md + 1, iy = c.ysz + 1, ix = c.xsz + 1;
-  std::vector<real> node[4];
-  for (auto& a : node) a.assign((size_t)nd * nd, real(0));
-  const std::vector<real>* src[4] = {&hr, &qxr, &qyr, &zr};
-  for (int y = 0; y < nd; ++y)
-    for (int x = 0; x < nd; ++x) {
-      const int i = (x == ix - 1 || x == nd - 1) ? x - 1 : x;
-      const int j = (y == iy - 1 || y == nd - 1) ? y - 1 : y;
-      const size_t ii = (size_t)(nd - 1 - y) * nd + x, mi = (size_t)(md - 1 - j) * md + i;
-      for (int v = 0; v < 4; ++v) node[v][ii] = (*src[v])[mi];
-    }
-  project_nodes<real>(c, md, node);
+  // the raster cells, rows bottom-up; the device builds the modal values
+  // (input/read_and_project_modes_{fv1,dg2}.cu, mra/modal_projections.cu)
+  c.raster[0].assign(N, real(0));
+  c.raster[3].assign(N, c.wall_height);
+  const std::string start = beside(par, par_str(par, "startfile"));
+  if (!start.empty()) read_raster<real>(start, c.raster[0], md, c.wall_height);
+  read_raster<real>(dem, c.raster[3], md, c.wall_height);
+  if (c.startq2d && !start.empty()) {
+    c.raster[1].assign(N, real(0));
+    c.raster[2].assign(N, real(0));
+    read_raster<real>(start + ".Qx", c.raster[1], md, c.wall_height);
+    read_raster<real>(start + ".Qy", c.raster[2], md, c.wall_height);
+  }
 }
 
 template void read_case<float>(const char*, int, const char* const*, HostCase<float>&);

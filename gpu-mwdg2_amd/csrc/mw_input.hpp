@@ -66,7 +66,12 @@ struct HostCase {
   HostBoundary<real> bdy[4];          // N, E, S, W
   std::vector<HostSource<real>> srcs;
   std::vector<int> stage_codes;
-  // row-major modal state: h0 h1x h1y qx0 qx1x qx1y qy0 qy1x qy1y z0 z1x z1y
+  // raster cells h, qx, qy, z (rows bottom-up, mesh_dim wide; qx/qy empty
+  // without startq2d), or for a synthetic case its 1D depths
+  std::vector<real> raster[4];
+  real syn_hl = 0, syn_hr = 0;
+  // row-major modal state given through mw_set_initial (else empty):
+  // h0 h1x h1y qx0 qx1x qx1y qy0 qy1x qy1y z0 z1x z1y
   std::vector<real> modal[12];
 };
 

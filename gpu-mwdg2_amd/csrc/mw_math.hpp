@@ -124,6 +124,82 @@ template <> MW_HD float exp_cw<float>(float x) {
   return (p * __builtin_bit_cast(float, (uint32_t)(e + 25 + 127) << 23)) * 2.98023223876953125e-08f;   // 2^-25
 }
 
+// Built-in synthetic cases: bed elevation and depth at a node (x, y)
+// (synthetic/topo*.cuh, synthetic/h_init*.cuh); hl, hr are the case's 1D
+// depths (classes/Depths1D.h). The c-property depth is the surface level
+// minus the bed, negative above it, as the reference returns.
+template <typename real>
+MW_HD real c_prop_bed_1d(real a) {                              // topo_c_prop_1D.cuh
+  real z = real(0.0);
+  if (a >= real(22) && a < real(25)) z = real(0.05) * a - real(1.1);
+  else if (a >= real(25) && a <= real(28)) z = real(-0.05) * a + real(1.4);
+  else if (a > real(8) && a < real(12)) z = real(0.2) - real(0.05) * (a - real(10)) * (a - real(10));
+  else if (a > real(39) && a < real(46.5)) z = real(0.3);
+  return real(10.0) * z;
+}
+template <typename real>
+MW_HD real triangle_bed(real a) {                               // topo_triangle.cuh
+  const real slope = real(0.4) / real(3.0);
+  if (a > real(25.5) && a <= real(28.5)) return (a - real(25.5)) * slope;
+  if (a > real(28.5) && a <= real(31.5)) return real(0.4) - (a - real(28.5)) * slope;
+  return real(0.0);
+}
+template <typename real>
+MW_HD real node_dist(real x, real y, real x0, real y0) {
+  return rsqrt_cr<real>((x - x0) * (x - x0) + (y - y0) * (y - y0));
+}
+template <typename real>
+MW_HD real synthetic_bed(real x, real y, int tc) {
+  switch (tc) {
+    case 1: case 3: case 11: case 13: return c_prop_bed_1d<real>(x);
+    case 2: case 4: case 12: case 14: return c_prop_bed_1d<real>(y);
+    case 15: return triangle_bed<real>(x);
+    case 16: return triangle_bed<real>(y);
+    case 17: return real(0.01) * x * x;                         // topo_parabolic_bowl.cuh
+    case 18: return real(0.01) * y * y;
+    case 19: case 20: {                                         // topo_three_cones.cuh
+      const real z1 = (real(8) - node_dist<real>(x, y, real(30), real(6))) / real(8);
+      const real z2 = (real(8) - node_dist<real>(x, y, real(30), real(24))) / real(8);
+      const real z3 = real(0.3) * (real(10) - node_dist<real>(x, y, real(47.5), real(15)));
+      return rmax<real>(rmax<real>(rmax<real>(z1, z2), z3), real(0.0));
+    }
+    case 21: {                                                  // topo_diff.cuh
+      real z = real(0);
+      z = rmax<real>(z, real(1.0) - real(0.2) * node_dist<real>(x, y, real(20.0), real(15.0)));
+      z = rmax<real>(z, real(2.0) - real(0.5) * node_dist<real>(x, y, real(40.0), real(15.0)));
+      return rmax<real>(z, real(3.0) - real(0.3) * node_dist<real>(x, y, real(60.0), real(15.0)));
+    }
+    case 22: {                                                  // topo_non_diff.cuh
+      const bool yb = real(11.0) <= y && y <= real(19.0);
+      if (real(16.0) <= x && x <= real(24.0) && yb) return real(0.86);
+      if (real(36.0) <= x && x <= real(44.0) && yb) return real(1.78);
+      if (real(56.0) <= x && x <= real(64.0) && yb) return real(2.30);
+      return real(0);
+    }
+    default: return real(0.0);
+  }
+}
+template <typename real>
+MW_HD real synthetic_depth(real x, real y, real z, real hl, real hr, int tc) {
+  switch (tc) {
+    case 1: case 2: case 3: case 4: return hl - z;
+    case 5: case 7: case 9: case 11: case 13:                  // h_init_overtopping.cuh
+      return (x <= real(25)) ? rmax<real>(real(0.0), hl - z) : rmax<real>(real(0.0), hr - z);
+    case 6: case 8: case 10: case 12: case 14:
+      return (y <= real(25)) ? rmax<real>(real(0.0), hl - z) : rmax<real>(real(0.0), hr - z);
+    case 15: return (x < real(15.5)) ? real(0.75) : real(0.0);
+    case 16: return (y < real(15.5)) ? real(0.75) : real(0.0);
+    case 17: return rmax<real>(real(0.0), (real(0.2) * x + real(10.0)) - z);
+    case 18: return rmax<real>(real(0.0), (real(0.2) * y + real(10.0)) - z);
+    case 19: return real(1.0) - z;
+    case 20: return (x < real(16.0)) ? rmax<real>(real(1.875) - z, real(0.0)) : real(0.0);
+    case 21: return real(1.95) - z;
+    case 22: return real(1.78) - z;
+    case 23: return (rsqrt_cr<real>(x * x + y * y) < real(2.5)) ? real(2.5) : real(0.5);
+    default: return real(0.0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Multiwavelet filter bank. The analysis map of one variable is a 12x12
 // orthogonal matrix acting on the 4 children's (0, 1x, 1y) coefficients; its
