@@ -1,5 +1,6 @@
 // Context, step orchestration and C ABI of the MI355X adaptive step.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <array>
@@ -77,6 +78,19 @@ struct CtxBase {
     MW_HIP(hipEventCreate(&e));
     return e;
   }
+  // Kernel launches go through launch_k: when a timed() section is open, the
+  // dispatch itself records the section's start and stop events
+  // (hipExtLaunchKernelGGL), so timing adds no packets between kernels.
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  template <typename... KArgs, typename... Args>
+  void launch_k(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
+    if (ev_a) {
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, ev_a, ev_b, 0, args...);
+      ev_a = ev_b = nullptr;
+    } else {
+      hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+  }
   template <typename F>
   void timed(const char* name, F&& launch) {
     const bool update = !std::strncmp(name, "stage", 5) || !std::strncmp(name, "fv1", 3) ||
@@ -84,9 +98,14 @@ struct CtxBase {
                         !std::strncmp(name, "emit_sweep", 10);
     if (!profiling || (profiling == 2 && !update)) { launch(); return; }
     hipEvent_t a = ev(), b = ev();
-    MW_HIP(hipEventRecord(a, stream));
+    ev_a = a;
+    ev_b = b;
     launch();
-    MW_HIP(hipEventRecord(b, stream));
+    if (ev_a) {                          // the section launched nothing
+      MW_HIP(hipEventRecord(a, stream));
+      MW_HIP(hipEventRecord(b, stream));
+      ev_a = ev_b = nullptr;
+    }
     pending.push_back({name, {a, b}});
   }
   void drain_profile() {
@@ -353,9 +372,9 @@ struct Ctx : CtxBase {
     int* hist = dalloc<int>((size_t)256 * tiles);
     for (int pass = 0; pass < 4; ++pass) {
       const int shift = 8 * pass;
-      hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, stream, k0, n, shift, hist, tiles);
-      hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, stream, hist, 256 * tiles);
-      hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(WG), 0, stream, k0, v0, k1, v1, n, shift, hist, tiles);
+      launch_k(k_radix_hist, dim3(tiles), dim3(WG), 0, stream, k0, n, shift, hist, tiles);
+      launch_k(k_radix_scan, dim3(1), dim3(1024), 0, stream, hist, 256 * tiles);
+      launch_k(k_radix_scatter, dim3(tiles), dim3(WG), 0, stream, k0, v0, k1, v1, n, shift, hist, tiles);
       std::swap(k0, k1);
       std::swap(v0, v1);
     }
@@ -370,10 +389,10 @@ struct Ctx : CtxBase {
     int* idx = dalloc<int>(N);
     int* k1 = dalloc<int>(N);
     int* v1 = dalloc<int>(N);
-    hipLaunchKernelGGL(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);
+    launch_k(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);
     sort_pairs(N, codes, idx, k1, v1);                    // idx now holds rev_z_order
     MW_HIP(hipMemcpyAsync(rev_z, idx, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
-    hipLaunchKernelGGL(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);  // idx = iota
+    launch_k(k_iota_morton, dim3(nblk(N)), dim3(WG), 0, stream, codes, idx, md, N);  // idx = iota
     int* keys = dalloc<int>(N);
     MW_HIP(hipMemcpyAsync(keys, rev_z, sizeof(int) * N, hipMemcpyDeviceToDevice, stream));
     sort_pairs(N, keys, idx, k1, v1);                     // idx now holds rev_row_major
@@ -398,10 +417,10 @@ struct Ctx : CtxBase {
       real** dtab = dalloc<real*>(12);
       MW_HIP(hipMemcpyAsync(dtab, dm.data(), sizeof(real*) * 12, hipMemcpyHostToDevice, stream));
       if (mwm)
-        hipLaunchKernelGGL((k_set_finest<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab,
+        launch_k((k_set_finest<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, (const real* const*)dtab,
                            rev_z, hier[0], zh, L, N, h0);
       else
-        hipLaunchKernelGGL((k_set_finest<real, false>), dim3(nblk(N)), dim3(WG), 0, stream,
+        launch_k((k_set_finest<real, false>), dim3(nblk(N)), dim3(WG), 0, stream,
                            (const real* const*)dtab, rev_z, hier[0], zh, L, N, h0);
       MW_HIP(hipGetLastError());
       MW_HIP(hipStreamSynchronize(stream));
@@ -424,9 +443,9 @@ struct Ctx : CtxBase {
       S.xmin = hc.xmin; S.ymin = hc.ymin; S.dx = hc.dx; S.hl = hc.syn_hl; S.hr = hc.syn_hr;
       S.s3 = std::sqrt(real(3.0));
       if (mwm)
-        hipLaunchKernelGGL((k_init_cells<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
+        launch_k((k_init_cells<real, true>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
       else
-        hipLaunchKernelGGL((k_init_cells<real, false>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
+        launch_k((k_init_cells<real, false>), dim3(nblk(N)), dim3(WG), 0, stream, S, hier[0], zh, L, h0);
       MW_HIP(hipGetLastError());
       MW_HIP(hipStreamSynchronize(stream));
       for (real* p : dr) (void)hipFree(p);
@@ -444,7 +463,7 @@ struct Ctx : CtxBase {
         MW_HIP(hipMemcpyAsync(dcodes, codes.data(), sizeof(int) * codes.size(), hipMemcpyHostToDevice, stream));
       MW_HIP(hipMemsetAsync(preflag, 0, D + 4, stream));
       if (work > 0)
-        hipLaunchKernelGGL(k_preflag_init, dim3(nblk(work)), dim3(WG), 0, stream, preflag, dcodes, (int)codes.size(), L,
+        launch_k(k_preflag_init, dim3(nblk(work)), dim3(WG), 0, stream, preflag, dcodes, (int)codes.size(), L,
                            thick, hc.xsz, hc.ysz);
       MW_HIP(hipGetLastError());
       MW_HIP(hipStreamSynchronize(stream));
@@ -471,12 +490,12 @@ struct Ctx : CtxBase {
       const int nparts = std::min(nblk(N), 1024);
       real* part = dalloc<real>((size_t)4 * nparts);
       if (mwm)
-        hipLaunchKernelGGL((k_init_maxima<real, true>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
+        launch_k((k_init_maxima<real, true>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
                            sources(), real(0.001), hc.dx, part);
       else
-        hipLaunchKernelGGL((k_init_maxima<real, false>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
+        launch_k((k_init_maxima<real, false>), dim3(nparts), dim3(WG), 0, stream, h0, hier[0], zh, L,
                            sources(), real(0.001), hc.dx, part);
-      hipLaunchKernelGGL((k_init_maxima_final<real>), dim3(1), dim3(64), 0, stream, st, part, nparts);
+      launch_k((k_init_maxima_final<real>), dim3(1), dim3(64), 0, stream, st, part, nparts);
       MW_HIP(hipGetLastError());
       MW_HIP(hipStreamSynchronize(stream));
       (void)hipFree(part);
@@ -486,10 +505,10 @@ struct Ctx : CtxBase {
     // z hierarchy and topography preflags (mra/preflag_topo.cu:3-43)
     for (int level = L - 1; level >= 0; --level) {
       if (mwm)
-        hipLaunchKernelGGL((k_z_encode<real, true>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+        launch_k((k_z_encode<real, true>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
                            level, L, hc.epsilon, (const DevState<real>*)st);
       else
-        hipLaunchKernelGGL((k_z_encode<real, false>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
+        launch_k((k_z_encode<real, false>), dim3(nblk(1LL << (2 * level))), dim3(WG), 0, stream, zh, preflag,
                            level, L, hc.epsilon, (const DevState<real>*)st);
     }
     MW_HIP(hipGetLastError());
@@ -504,14 +523,14 @@ struct Ctx : CtxBase {
   // ---- step pieces ---------------------------------------------------------
   void k_prologue_launch() {
     timed("prologue", [&] {
-      hipLaunchKernelGGL((k_prologue<real>), dim3(1), dim3(PRO_WG), 0, stream, st, P, series(), sources(), red,
+      launch_k((k_prologue<real>), dim3(1), dim3(PRO_WG), 0, stream, st, P, series(), sources(), red,
                          2 * leaf_grid());
     });
   }
   void k_sources_launch() {
     if (!n_src) return;
     timed("point_sources", [&] {
-      hipLaunchKernelGGL((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx);
+      launch_k((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx);
     });
   }
   int top_level() const { return std::min(TOP_LEVEL, L - 1); }
@@ -533,19 +552,19 @@ struct Ctx : CtxBase {
       const int nwg = 1 << (2 * b.first);
       timed("encode", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_encode_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+          launch_k((k_encode_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                              xflag, preflag, b.first, b.second, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
         else
-          hipLaunchKernelGGL((k_encode_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+          launch_k((k_encode_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                              xflag, preflag, b.first, b.second, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       });
     }
     timed("encode_top", [&] {
       if (mwm)
-        hipLaunchKernelGGL((k_encode_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+        launch_k((k_encode_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            preflag, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       else
-        hipLaunchKernelGGL((k_encode_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+        launch_k((k_encode_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            preflag, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0]);
     });
   }
@@ -554,10 +573,10 @@ struct Ctx : CtxBase {
     uint8_t* nw = sig[cur ^ 1];
     timed("decode_top", [&] {
       if (mwm)
-        hipLaunchKernelGGL((k_topdown_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+        launch_k((k_topdown_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
       else
-        hipLaunchKernelGGL((k_topdown_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
+        launch_k((k_topdown_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
     });
     const auto bs = bands();
@@ -565,10 +584,10 @@ struct Ctx : CtxBase {
       const int nwg = 1 << (2 * it->first);
       timed("decode", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_topdown_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+          launch_k((k_topdown_band<real, true>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                              xflag, leaflev, it->first, it->second, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
         else
-          hipLaunchKernelGGL((k_topdown_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+          launch_k((k_topdown_band<real, false>), dim3(nwg), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                              xflag, leaflev, it->first, it->second, L, hc.epsilon, P.eps_extra_factor, cnt[1]);
       });
     }
@@ -582,20 +601,20 @@ struct Ctx : CtxBase {
     const int tiles = emit_tiles();
     int* done = &st->done;
     int* nl = &st->n_leaves;
-    timed("emit_count", [&] { hipLaunchKernelGGL(k_emit_count, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums); });
-    timed("emit_scan", [&] { hipLaunchKernelGGL(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles,
+    timed("emit_count", [&] { launch_k(k_emit_count, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums); });
+    timed("emit_scan", [&] { launch_k(k_emit_scan, dim3(1), dim3(1024), 0, stream, done, nl, tile_sums, tiles,
                                                   &st->clk_step, &st->mra_ticks); });
     if (sweep) {
       WetList W = wet[0];
       W.cap = 4 * EMIT_TILE;
       timed("emit_sweep1", [&] {
-        hipLaunchKernelGGL((k_emit_sweep1<real>), dim3(tiles), dim3(WG), 0, stream, st, P, leaflev, Q, tile_sums, leaf,
+        launch_k((k_emit_sweep1<real>), dim3(tiles), dim3(WG), 0, stream, st, P, leaflev, Q, tile_sums, leaf,
                            nbr[0], nbr[1], nbr[2], nbr[3], hier[main_h], hier[main_h ^ 1], zh, W);
       });
       return;
     }
     timed("emit_write", [&] {
-      hipLaunchKernelGGL(k_emit_write, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums, leaf, nbr[0],
+      launch_k(k_emit_write, dim3(tiles), dim3(WG), 0, stream, done, leaflev, L, Q, tile_sums, leaf, nbr[0],
                          nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
     });
   }
@@ -605,8 +624,8 @@ struct Ctx : CtxBase {
     WetList W = wet[q];
     W.cap = cap;
     timed("wet_list", [&] {
-      hipLaunchKernelGGL(k_wet_scan, dim3(1), dim3(1024), 0, stream, &st->done, W, nwg);
-      hipLaunchKernelGGL(k_wet_gather, dim3(nwg), dim3(WG), 0, stream, &st->done, W);
+      launch_k(k_wet_scan, dim3(1), dim3(1024), 0, stream, &st->done, W, nwg);
+      launch_k(k_wet_gather, dim3(nwg), dim3(WG), 0, stream, &st->done, W);
     });
   }
   // swept: the stage-1 sweep already ran inside the leaf emission
@@ -620,23 +639,23 @@ struct Ctx : CtxBase {
         wet_list_launch(0, nwg, 4 * EMIT_TILE);
       } else {
         timed("sweep1", [&] {
-          hipLaunchKernelGGL((k_sweep1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other],
+          launch_k((k_sweep1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other],
                              zh, wet[0]);
         });
         wet_list_launch(0, G, wet[0].cap);
       }
       timed("wet1", [&] {
-        hipLaunchKernelGGL((k_wet1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other], zh,
+        launch_k((k_wet1<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h], hier[other], zh,
                            wet[0], nwg);
       });
       return;
     }
     timed(mwm ? "stage1" : "fv1", [&] {
       if (mwm)
-        hipLaunchKernelGGL((k_stage1<real, true, true>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h],
+        launch_k((k_stage1<real, true, true>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[main_h],
                            hier[other], zh, uprev, red);
       else
-        hipLaunchKernelGGL((k_stage1<real, false, false>), dim3(G), dim3(WG), 0, stream, st, P, leaves(),
+        launch_k((k_stage1<real, false, false>), dim3(G), dim3(WG), 0, stream, st, P, leaves(),
                            hier[main_h], hier[other], zh, uprev, red);
     });
     if (!mwm) main_h = other;
@@ -645,12 +664,12 @@ struct Ctx : CtxBase {
     for (const auto& b : bands()) {
       const int nwg = 1 << (2 * b.first);
       timed("encode_scale", [&] {
-        hipLaunchKernelGGL((k_encode_scale_band<real>), dim3(nwg), dim3(WG), 0, stream, st, hier[1], sig[cur], b.first,
+        launch_k((k_encode_scale_band<real>), dim3(nwg), dim3(WG), 0, stream, st, hier[1], sig[cur], b.first,
                            b.second, cnt[2]);
       });
     }
     timed("encode_scale_top", [&] {
-      hipLaunchKernelGGL((k_encode_scale_top<real>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[1], sig[cur], top_level(),
+      launch_k((k_encode_scale_top<real>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[1], sig[cur], top_level(),
                          cnt[2]);
     });
   }
@@ -659,21 +678,21 @@ struct Ctx : CtxBase {
     if (hc.limitslopes) {
       MW_HIP(hipMemsetAsync(maxh2, 0, sizeof(unsigned long long), stream));
       timed("maxh2", [&] {
-        hipLaunchKernelGGL((k_maxh_stage2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[1], zh, maxh2);
+        launch_k((k_maxh_stage2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[1], zh, maxh2);
       });
       timed("stage2", [&] {
-        hipLaunchKernelGGL((k_stage2_limited<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
+        launch_k((k_stage2_limited<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1],
                            zh, uprev, maxh2, red);
       });
       return;
     }
     timed("sweep2", [&] {
-      hipLaunchKernelGGL((k_sweep2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1],
+      launch_k((k_sweep2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1],
                          red);
     });
     wet_list_launch(1, G, wet[1].cap);
     timed("wet2", [&] {
-      hipLaunchKernelGGL((k_wet2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1], G,
+      launch_k((k_wet2<real>), dim3(G), dim3(WG), 0, stream, st, P, leaves(), hier[0], hier[1], zh, wet[1], G,
                          red);
     });
   }
@@ -698,10 +717,10 @@ struct Ctx : CtxBase {
     if (rec || cum) {
       timed("stage_record", [&] {
         if (mwm)
-          hipLaunchKernelGGL((k_stage_record<real, true>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
+          launch_k((k_stage_record<real, true>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
                              leaflev, stage_codes, n_stage, rec, cum, rec_cap, red, 2 * leaf_grid());
         else
-          hipLaunchKernelGGL((k_stage_record<real, false>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
+          launch_k((k_stage_record<real, false>), dim3(1), dim3(256), 0, stream, st, P, hier[main_h], zh,
                              leaflev, stage_codes, n_stage, rec, cum, rec_cap, red, 2 * leaf_grid());
       });
     }
@@ -735,10 +754,10 @@ struct Ctx : CtxBase {
     real* o[6];
     for (int k = 0; k < 6; ++k) o[k] = rbuf + (size_t)k * n;
     if (mwm)
-      hipLaunchKernelGGL((k_raster_project<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh, leaflev,
+      launch_k((k_raster_project<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh, leaflev,
                          o[0], o[1], o[2], o[3], o[4], o[5]);
     else
-      hipLaunchKernelGGL((k_raster_project<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh,
+      launch_k((k_raster_project<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, P, hier[main_h], zh,
                          leaflev, o[0], o[1], o[2], o[3], o[4], o[5]);
     MW_HIP(hipGetLastError());
     std::vector<real> h((size_t)6 * n);
@@ -844,10 +863,10 @@ struct Ctx : CtxBase {
     F.n = (int)ids.size();
     for (int k = 0; k < F.n; ++k) F.v[k] = ids[k];
     if (mwm)
-      hipLaunchKernelGGL((k_leaf_fields<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
+      launch_k((k_leaf_fields<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
                          F, lbuf, N);
     else
-      hipLaunchKernelGGL((k_leaf_fields<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
+      launch_k((k_leaf_fields<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, st, hier[main_h], zh, leaf,
                          F, lbuf, N);
     MW_HIP(hipGetLastError());
     std::vector<real> out((size_t)F.n * n);
@@ -929,7 +948,7 @@ struct Ctx : CtxBase {
     }
     ++cprop_writes;
     if (!clk) clk = alloc<double>(2);
-    hipLaunchKernelGGL((k_clock<real>), dim3(1), dim3(64), 0, stream, st, clk);
+    launch_k((k_clock<real>), dim3(1), dim3(64), 0, stream, st, clk);
     MW_HIP(hipGetLastError());
     double c[2];
     MW_HIP(hipMemcpyAsync(c, clk, sizeof c, hipMemcpyDeviceToHost, stream));
@@ -1071,10 +1090,10 @@ struct Ctx : CtxBase {
 
   void coef_io_launch(int b, int which, int n, bool to_buf) {
     if (mwm)
-      hipLaunchKernelGGL((k_coef_io<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
+      launch_k((k_coef_io<real, true>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
                          to_buf ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_coef_io<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
+      launch_k((k_coef_io<real, false>), dim3(nblk(n)), dim3(WG), 0, stream, hier[b], zh, which, xbuf, n,
                          to_buf ? 1 : 0);
     MW_HIP(hipGetLastError());
   }
