@@ -14,7 +14,7 @@
 // bit for bit. Two deliberate, documented deviations:
 //   * h^(1/3) (friction, FREE ghost) is evaluated with cbrt_newton() below
 //     (a fixed Newton iteration from a bit-level seed) instead of pow(h,1/3),
-//     so host and device produce identical bits; the two agree to ~1 ulp.
+//     so host and device produce identical bits; within a few ulp of cbrt.
 //   * time-series lookup holds the last value once the row index would step
 //     past the table end (the reference reads past the array there,
 //     classes/Boundary.h:476-496).
@@ -72,32 +72,38 @@ inline int compact(int code) {
 inline int morton(int x, int y) { return dilate(x) | (dilate(y) << 1); }
 
 // Cube root used in place of pow(x, 1/3) (apply_friction.cuh:41,
-// add_ghost_cells.cu:135). Seed from the exponent bits, then Newton steps
-// y <- (2y + x/y^2)/3. Same operations, same order on host and device.
+// add_ghost_cells.cu:135), with no division: Newton on r = x^(-1/3),
+// r <- r (4 - x r^3) / 3, from a seed built from the exponent bits
+// ((4/3) * bias - bits / 3); then y = x r^2 and one correction
+// y <- y - (y^3 - x) r^2 / 3. Same operations, same order on host and device.
 template <typename real> inline real cbrt_newton(real x);
 template <> inline double cbrt_newton<double>(double x) {
   if (!(x > 0.0)) return (x == 0.0) ? 0.0 : std::nan("");
   if (std::isinf(x)) return x;
   uint64_t b; std::memcpy(&b, &x, 8);
   const bool sub = (b >> 52) == 0;
-  double xs = sub ? x * 18014398509481984.0 : x;  // 2^54 for subnormals
+  const double xs = sub ? x * 18014398509481984.0 : x;   // 2^54 for subnormals
   std::memcpy(&b, &xs, 8);
-  b = b / 3 + 0x2A9F7893782DA1CEull;
-  double y; std::memcpy(&y, &b, 8);
-  for (int it = 0; it < 6; ++it) y = (2.0 * y + xs / (y * y)) / 3.0;
-  return sub ? y / 262144.0 : y;                    // 2^18 = cbrt(2^54)
+  b = 0x5540000000000000ull - b / 3;
+  double r; std::memcpy(&r, &b, 8);
+  for (int it = 0; it < 4; ++it) r = r * (4.0 - xs * r * r * r) * (1.0 / 3.0);
+  double y = xs * r * r;
+  y = y - (y * y * y - xs) * (r * r) * (1.0 / 3.0);
+  return sub ? y * (1.0 / 262144.0) : y;                 // 2^18 = cbrt(2^54)
 }
 template <> inline float cbrt_newton<float>(float x) {
   if (!(x > 0.0f)) return (x == 0.0f) ? 0.0f : std::nanf("");
   if (std::isinf(x)) return x;
   uint32_t b; std::memcpy(&b, &x, 4);
   const bool sub = (b >> 23) == 0;
-  float xs = sub ? x * 16777216.0f : x;             // 2^24 for subnormals
+  const float xs = sub ? x * 16777216.0f : x;             // 2^24 for subnormals
   std::memcpy(&b, &xs, 4);
-  b = b / 3 + 0x2A51067Fu;
-  float y; std::memcpy(&y, &b, 4);
-  for (int it = 0; it < 5; ++it) y = (2.0f * y + xs / (y * y)) / 3.0f;
-  return sub ? y / 256.0f : y;                      // 2^8 = cbrt(2^24)
+  b = 0x54AAAAABu - b / 3;
+  float r; std::memcpy(&r, &b, 4);
+  for (int it = 0; it < 3; ++it) r = r * (4.0f - xs * r * r * r) * (1.0f / 3.0f);
+  float y = xs * r * r;
+  y = y - (y * y * y - xs) * (r * r) * (1.0f / 3.0f);
+  return sub ? y * (1.0f / 256.0f) : y;                   // 2^8 = cbrt(2^24)
 }
 
 // e^x for the Oregon wavemaker pulse: Cody-Waite reduction x = n ln2 + r
@@ -134,6 +140,7 @@ template <> inline float exp_cw<float>(float x) {
   if (e >= -126) return p * pow2(e);
   return (p * pow2(e + 25)) * pow2(-25);
 }
+
 
 // ---------------------------------------------------------------------------
 // Input parsing: a restatement of the keyword readers (prefix match of the

@@ -344,3 +344,24 @@ def test_c_property_lake_at_rest(tmp_path, solver, tc):
         assert worst < 1e-10, worst
     finally:
         o.close()
+
+
+def test_cube_root_against_libm():
+    """The cube root that stands in for pow(h, 1/3) in the Manning term and the
+    FREE ghost depth (operators/apply_friction.cuh:41,
+    neighbours/add_ghost_cells.cu:135): within 4 ulp of numpy's cbrt in FP64
+    and 2 ulp of the exact value in FP32, over depths from 1e-12 m to 1e4 m,
+    subnormals, zero, and the error values."""
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([10.0 ** rng.uniform(-12, 4, 6000), rng.uniform(1e-3, 3.0, 3000),
+                         [1.0, 8.0, 27.0, 1e-310, 5e-324, 1e300]])
+    for x in xs:
+        got, ref = oracle.cbrt(float(x)), float(np.cbrt(x))
+        assert abs(got - ref) <= 4 * np.spacing(ref), (x, got, ref)
+        if x < 3e38:
+            xf = np.float32(x)
+            gf, rf = np.float32(oracle.cbrt(float(xf), np.float32)), np.cbrt(np.float64(xf))
+            if xf > 0:
+                assert abs(float(gf) - rf) <= 2 * float(np.spacing(np.float32(rf))), (x, gf, rf)
+    assert oracle.cbrt(0.0) == 0.0 and np.isnan(oracle.cbrt(-1.0))
+    assert oracle.cbrt(8.0) == 2.0 and oracle.cbrt(27.0, np.float32) == 3.0
