@@ -1,0 +1,156 @@
+"""The GPU path at the headline workload's full size (max_ref_lvl 12, 4096 x
+4096 finest cells), checked by properties that hold at any size; the oracle
+pins the same code bit for bit at smaller sizes (test_gpu_parity.py).
+
+- Leaf list (traverse_tree_of_sig_details.cu, compaction.cu:109-151): the
+  leaves tile the 4^12 finest cells once, in increasing Morton order, every
+  ancestor of a leaf is significant and a leaf above the finest level is not.
+- Neighbours (find_neighbours.cu:87-108): brute force over a sample of leaves
+  from the finest-cell -> leaf map: N/E from the leaf's last finest cell, S/W
+  from its first, the node at the coarser of the two levels, -1 at the edge.
+- Lake at rest (cases/tests/test.py:105, 396) over the 1D bed of case 1 (wet
+  everywhere) and the blocks of case 22 (wet and dry): every discharge
+  coefficient stays below 1e-10 in FP64. (Case 21, the smooth cones, keeps
+  this only up to max_ref_lvl 8: from 9 on the oracle itself reaches 1e-7 to
+  1e-6 within 10 steps at the wet/dry cone tops, so it is not a full-size
+  check.)
+- Closed walls on a uniform grid (eps = 0), the circular dam break (wet
+  everywhere): the water volume is conserved to round-off in FP64 (the
+  fluxes telescope; with dry cells the depth and discharge cut-offs of the
+  method move volume, in the oracle as well).
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+L = 12
+
+
+def lvl(l):
+    return ((1 << (2 * l)) - 1) // 3 + 3
+
+
+def level_of(h):
+    return np.floor(np.log2(3 * (h.astype(np.int64) - 3) + 1) / 2).astype(np.int64)
+
+
+def dilate(v):
+    v = v.astype(np.int64) & 0xFFFF
+    v = (v | (v << 8)) & 0x00FF00FF
+    v = (v | (v << 4)) & 0x0F0F0F0F
+    v = (v | (v << 2)) & 0x33333333
+    v = (v | (v << 1)) & 0x55555555
+    return v
+
+
+def compact(c):
+    c = c.astype(np.int64) & 0x55555555
+    c = (c | (c >> 1)) & 0x33333333
+    c = (c | (c >> 2)) & 0x0F0F0F0F
+    c = (c | (c >> 4)) & 0x00FF00FF
+    c = (c | (c >> 8)) & 0x0000FFFF
+    return c
+
+
+@pytest.fixture(scope="module")
+def urban12(tmp_path_factory):
+    import mwdg2
+    par = cases.urban(str(tmp_path_factory.mktemp("urban12")), L=L, t_end=1800.0)
+    s = mwdg2.Simulation(par, "f32")
+    s.step(20)
+    out = {"leaves": s.get("leaves").astype(np.int64), "sig": s.get("sig"),
+           "nbr": [s.get(f"nbr{k}").astype(np.int64) for k in range(4)], "c": s.counters(),
+           "eta0": s.get("A.eta0"), "qx0": s.get("A.qx0"), "z0": s.get("z0")}
+    s.close()
+    return out
+
+
+def test_leaves_tile_the_grid_in_morton_order(urban12):
+    leaves = urban12["leaves"]
+    lev = level_of(leaves)
+    code = leaves - np.array([lvl(int(l)) for l in range(L + 1)])[lev]
+    size = 4 ** (L - lev)
+    start = code * size
+    assert leaves.size == urban12["c"]["n_leaves"] and leaves.size > 1_000_000
+    assert start[0] == 0 and int(size.sum()) == 4 ** L
+    assert np.array_equal(start[1:], start[:-1] + size[:-1])
+
+
+def test_tree_is_closed(urban12):
+    leaves, sig = urban12["leaves"], urban12["sig"]
+    lev = level_of(leaves)
+    code = leaves - np.array([lvl(int(l)) for l in range(L + 1)])[lev]
+    for k in range(L):
+        m = lev > k
+        anc = np.array(lvl(k)) + (code[m] >> (2 * (lev[m] - k)))
+        assert np.all(sig[anc] == 1), k
+    coarse = lev < L
+    assert np.all(sig[leaves[coarse]] == 0)
+
+
+def test_neighbours_brute_force_sample(urban12):
+    leaves = urban12["leaves"]
+    lev = level_of(leaves)
+    code = leaves - np.array([lvl(int(l)) for l in range(L + 1)])[lev]
+    size = 4 ** (L - lev)
+    owner = np.repeat(np.arange(leaves.size), size)          # finest Morton code -> leaf
+    rng = np.random.default_rng(5)
+    idx = rng.choice(leaves.size, 50000, replace=False)
+    first = code[idx] * size[idx]
+    last = first + size[idx] - 1
+    xs, ys = compact(first), compact(first >> 1)
+    xe, ye = compact(last), compact(last >> 1)
+    n = 1 << L
+    cand = {0: (xe, ye + 1, ye == n - 1), 1: (xe + 1, ye, xe == n - 1), 2: (xs, ys - 1, ys == 0),
+            3: (xs - 1, ys, xs == 0)}
+    for k, (xn, yn, border) in cand.items():
+        xn = np.clip(xn, 0, n - 1)
+        yn = np.clip(yn, 0, n - 1)
+        m = dilate(xn) | (dilate(yn) << 1)
+        j = owner[m]
+        ml = np.minimum(lev[idx], lev[j])
+        exp = np.array([lvl(int(l)) for l in range(L + 1)])[ml] + (m >> (2 * (L - ml)))
+        exp = np.where(border, -1, exp)
+        got = urban12["nbr"][k][idx]
+        assert np.array_equal(got, exp), (k, np.flatnonzero(got != exp)[:5])
+
+
+def test_state_is_finite_and_advancing(urban12):
+    c = urban12["c"]
+    leaves = urban12["leaves"]
+    assert c["steps"] == 20 and c["time"] > 0 and 0 < c["dt_next"] < 1.0
+    for name in ("eta0", "qx0", "z0"):
+        assert np.all(np.isfinite(urban12[name][leaves])), name
+    h = urban12["eta0"][leaves] - urban12["z0"][leaves]
+    assert h.max() > 1.0 and (h >= 1e-3).mean() < 0.5     # the wet strip, mostly dry plain
+
+
+@pytest.mark.parametrize("tc", [1, 22])
+def test_lake_at_rest_full_size(case_dir, tc):
+    import mwdg2
+    par = cases.synthetic(str(case_dir / f"cprop{tc}_l12"), tc, L=L, solver="mwdg2", t_end=100.0)
+    with mwdg2.Simulation(par, "f64") as s:
+        s.step(10)
+        leaves = s.get("leaves")
+        worst = max(float(np.abs(s.get("A." + f)[leaves]).max())
+                    for f in ("qx0", "qy0", "qx1x", "qx1y", "qy1x", "qy1y"))
+        n = s.counters()["n_leaves"]
+    assert n < 4 ** L
+    assert worst < 1e-10, worst
+
+
+def test_volume_conserved_uniform_full_size(case_dir):
+    import mwdg2
+    par = cases.circular_dam_break(str(case_dir / "mass_l12"), L=L, eps=0.0, t_end=3.5)
+    with mwdg2.Simulation(par, "f64") as s:
+        s.step(1)                      # the first step's decode rounds the initial state once
+        leaves = s.get("leaves")
+        assert leaves.size == 4 ** L
+        v0 = float(np.sum(s.get("A.eta0")[leaves] - s.get("z0")[leaves]))
+        s.step(10)
+        v1 = float(np.sum(s.get("A.eta0")[leaves] - s.get("z0")[leaves]))
+        assert s.counters()["steps"] == 11
+    assert v0 > 0 and abs(v1 - v0) <= 1e-11 * v0, (v0, v1)
