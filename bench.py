@@ -74,6 +74,30 @@ def kernel_bytes(s: int, L: int, leaves: int, wet1: int, wet2: int) -> dict:
     }
 
 
+def step_bytes_method(s: int, L: int, leaves: float, p_enc: float, p_dec: float, friction: bool) -> dict:
+    """Algorithmic HBM bytes of one adaptive MWDG2 step by pass of the method
+    (every logical element the step must read or write counted once, b = 1,
+    i = 4), whatever the kernels that implement the passes: leaves written
+    back into the hierarchy, encode + threshold, regularisation + extra
+    significance, decode, leaf list + neighbours, friction, the two RK stages
+    with dt, the stage-1 write-back and re-encode for neighbour reads."""
+    i, b = 4, 1
+    D = ((1 << (2 * L)) - 1) // 3 + 3
+    na, pe, pd = leaves, p_enc, p_dec
+    passes = {
+        "reinsert": na * (12 * s + i) + na * 9 * s,
+        "encode": pe * (36 * s + 36 * s + s + 3 * b),
+        "regularise_extra": D * 3 * b + pd * s,
+        "decode": pd * (36 * s + 36 * s),
+        "leaf_list_neighbours": D * b + na * 10 * i,
+        "friction": na * (9 * s + 6 * s) if friction else 0,
+        "stages": na * (12 * s + 9 * s + s) + na * (12 * s + 9 * s + 9 * s + s),
+        "reencode_for_neighbours": na * 9 * s + pe * (36 * s + 9 * s),
+    }
+    passes["total"] = sum(passes.values())
+    return passes
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` (read + write) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, each counter
@@ -228,6 +252,18 @@ def main():
             out_k[name] = {"launches": n, "avg_us": round(1000.0 * ms / n, 2), "ms_per_step": round(ms / max(1, steps), 4)}
         print(json.dumps({"per_kernel": out_k}), file=sys.stderr)
 
+    # the whole step against the roofline: the method's bytes per step over
+    # the step time (the kernels run back to back, so their times sum to it)
+    per = lambda k: (c1[k] - c0[k]) / max(1, steps)
+    sb = step_bytes_method(4, args.L, leaves / max(1, steps), per("parents_encoded"), per("parents_decoded"), True)
+    own = sum(v["alg_bytes_per_launch"] * v["launches"] for v in kern.values()) / max(1, steps)
+    step_gbs = sb["total"] / (ms_per_step * 1e-3) / 1e9
+    step_roof = {"alg_bytes_per_step": int(sb["total"]), "by_pass": {k: int(v) for k, v in sb.items() if k != "total"},
+                 "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4), "measured_copy_gbs": round(copy_gbs, 1),
+                 "frac_of_measured_copy": round(step_gbs / copy_gbs, 4),
+                 "update_kernels_alg_bytes_per_step": int(own)}
+
     cpu = None
     if not args.no_cpu:
         sim.close()
@@ -250,6 +286,7 @@ def main():
                    "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                for k, v in kern.items()}},
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     if acc is not None:
