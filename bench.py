@@ -130,12 +130,13 @@ def finest_fields(sim, L: int):
     return np.repeat(h0, reps), np.repeat(qx, reps), np.repeat(qy, reps), vol
 
 
-def accuracy_vs_fp64(par: str, L: int, steps: int, f32) -> dict:
-    """FP32 product against the FP64 product (bit-identical to the FP64
-    oracle) after the same number of steps of the same case."""
+def accuracy_vs_fp64(par: str, L: int, steps: int, runs: dict) -> dict:
+    """FP32 runs (plain, learned) against the plain FP64 product (bit-identical
+    to the FP64 oracle) after the same number of steps of the same case:
+    max, mean and 99.9th percentile of |dh| over the finest cells, max |dq|,
+    and the volume difference."""
     import numpy as np
     import mwdg2
-    h32, qx32, qy32, v32 = f32
     s64 = mwdg2.Simulation(par, "f64")
     fl = ((1 << (2 * L)) - 1) // 3 + 3           # first finest node: the initial state lives there
     v0 = float(np.sum(s64.get("A.eta0")[fl:].astype(np.float64) - s64.get("z0")[fl:]))
@@ -144,11 +145,88 @@ def accuracy_vs_fp64(par: str, L: int, steps: int, f32) -> dict:
     h64, qx64, qy64, v64 = finest_fields(s64, L)
     t64 = s64.counters()["time"]
     s64.close()
-    return {"steps": steps, "time_fp64": t64,
-            "max_abs_dh": float(np.max(np.abs(h32 - h64))), "max_abs_dqx": float(np.max(np.abs(qx32 - qx64))),
-            "max_abs_dqy": float(np.max(np.abs(qy32 - qy64))),
-            "volume_fp32": v32, "volume_fp64": v64, "volume_rel_diff": (v32 - v64) / v64,
-            "volume_change_fp64": v64 - v0}
+    out = {"steps": steps, "time_fp64": t64, "volume_fp64": v64, "volume_change_fp64": v64 - v0}
+    for name, (h, qx, qy, v, t) in runs.items():
+        d = np.abs(h - h64)
+        out[name] = {"time": t, "max_abs_dh": float(d.max()), "mean_abs_dh": float(d.mean()),
+                     "p999_abs_dh": float(np.quantile(d, 0.999)), "max_abs_dqx": float(np.max(np.abs(qx - qx64))),
+                     "max_abs_dqy": float(np.max(np.abs(qy - qy64))), "volume_rel_diff": (v - v64) / v64}
+    return out
+
+
+def learned_leg(par: str, L: int, warmup: int, steps: int, trace_steps: int) -> tuple:
+    """Trains the refinement network on the GPU from plain-step traces of two
+    other seeded urban layouts of the same size, checks it on a third, then
+    times the learned step on the benchmark case (and the step with every unit
+    frozen, i.e. the exact guard alone) over as many steps as the plain run."""
+    import mwdg2
+    import cases
+    base = os.environ.get("TMPDIR", "/tmp")
+    t0 = time.time()
+    m = mwdg2.Mlp(seed=7)
+    for seed in (4, 5):
+        d = os.path.join(base, f"mwdg2_bench_urban_L{L}_s{seed}")
+        tp = os.path.join(d, "case.par")
+        if not os.path.exists(tp):
+            cases.urban(d, L=L, seed=seed, src_seed=30 + seed, t_end=1800.0)
+        with mwdg2.Simulation(tp, "f32") as s:
+            m.record(s, trace_steps)
+    n = m.build_labels(8)
+    t_rec = time.time() - t0
+    t1 = time.time()
+    losses = m.fit(10, batch=4096, lr=2e-3, pos_weight=2.0)
+    t_fit = time.time() - t1
+    held = mwdg2.Mlp(seed=7)
+    d = os.path.join(base, f"mwdg2_bench_urban_L{L}_s6")
+    hp = os.path.join(d, "case.par")
+    if not os.path.exists(hp):
+        cases.urban(d, L=L, seed=6, src_seed=36, t_end=1800.0)
+    with mwdg2.Simulation(hp, "f32") as s:
+        held.record(s, trace_steps // 2)
+    held.build_labels(8)
+    untrained = held.params
+    held.params = m.params
+    ct = held.eval(0.5)
+    held.params = untrained
+    cu = held.eval(0.5)
+    held.close()
+
+    def f1(c):
+        p = c["tp"] / max(1, c["tp"] + c["fp"])
+        r = c["tp"] / max(1, c["tp"] + c["fn"])
+        return {"precision": round(p, 4), "recall": round(r, 4), "f1": round(2 * p * r / max(1e-12, p + r), 4)}
+
+    res = {"training": {"samples": n, "trace_steps": 2 * trace_steps, "record_s": round(t_rec, 2),
+                        "fit_s": round(t_fit, 3), "epochs": len(losses), "loss_first": float(losses[0]),
+                        "loss_last": float(losses[-1]), "held_out_trained": f1(ct), "held_out_untrained": f1(cu)}}
+    fields = None
+    for mode, thr in (("learned", 0.5), ("guard_only", 2.0)):
+        sim = mwdg2.Simulation(par, "f32")
+        sim.set_learned(m, decide_every=8, full_every=32, threshold=thr)
+        sim.step(warmup)
+        sim.sync()
+        c0 = sim.counters()
+        ta = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        tb = time.perf_counter()
+        c1 = sim.counters()
+        k = c1["steps"] - c0["steps"]
+        r = {"ms_per_step": 1000.0 * (tb - ta) / max(1, k), "value": (4 ** L) * k / (tb - ta)}
+        if mode == "learned":
+            sim.profile(2)
+            sim.step(steps)
+            sim.sync()
+            sim.profile(0)
+            lc = sim.learn_counters()
+            r["frozen_unit_fraction"] = round(lc["frozen_unit_steps"] / max(1, lc["steps"] * 4096), 4)
+            r["parents_kept_per_step"] = lc["parents_kept"] / max(1, lc["steps"])
+            h, qx, qy, v = finest_fields(sim, L)
+            fields = (h, qx, qy, v, sim.counters()["time"])
+        res[mode] = r
+        sim.close()
+    m.close()
+    return res, fields
 
 
 def run_cpu_baseline(par: str, L: int, budget_s: float) -> dict:
@@ -182,6 +260,8 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel (adds event overhead)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip comparing the FP32 state after the run with an FP64 run of the same steps")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned step")
+    ap.add_argument("--trace-steps", type=int, default=200, help="plain steps recorded per training layout")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -238,9 +318,7 @@ def main():
                 "alg_bytes_per_launch": int(kern[dom]["alg_bytes_per_launch"]),
                 "avg_launch_ms": round(kern[dom]["avg_ms"], 4)}
 
-    acc = None
-    if not args.no_accuracy:
-        acc = accuracy_vs_fp64(par, args.L, args.warmup + 2 * args.steps, finest_fields(sim, args.L))
+    plain_fields = finest_fields(sim, args.L) + (c2["time"],) if not args.no_accuracy else None
 
     copy_gbs = mwdg2.stream_copy_gbs(1 << 30, 20)
     if roof:
@@ -264,9 +342,21 @@ def main():
                  "frac_of_measured_copy": round(step_gbs / copy_gbs, 4),
                  "update_kernels_alg_bytes_per_step": int(own)}
 
+    sim.close()
+    learned, lf = (None, None)
+    if not args.no_learned and 7 <= args.L <= 12:
+        learned, lf = learned_leg(par, args.L, args.warmup, args.steps, args.trace_steps)
+        learned["speedup_vs_plain"] = round(ms_per_step / learned["learned"]["ms_per_step"], 4)
+        learned["guard_only"]["speedup_vs_plain"] = round(ms_per_step / learned["guard_only"]["ms_per_step"], 4)
+    acc = None
+    if not args.no_accuracy:
+        runs = {"plain": plain_fields}
+        if lf is not None:
+            runs["learned"] = lf
+        acc = accuracy_vs_fp64(par, args.L, args.warmup + 2 * args.steps, runs)
+
     cpu = None
     if not args.no_cpu:
-        sim.close()
         cpu = run_cpu_baseline(par, args.L, args.cpu_budget)
 
     out = {
@@ -289,6 +379,8 @@ def main():
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
+    if learned is not None:
+        out["learned"] = learned
     if acc is not None:
         out["accuracy_fp32_vs_fp64"] = acc
     print(json.dumps(out))

@@ -18,6 +18,7 @@
 #include "../../include/mwdg2.h"
 #include "mw_input.hpp"
 #include "mw_kernels.inc"
+#include "mw_learn.inc"
 
 namespace mw {
 
@@ -144,6 +145,11 @@ struct CtxBase {
   virtual void write_c_prop() = 0;
   virtual void save_outputs(int index) = 0;
   virtual bool saveint_outputs() const = 0;
+  // learned refinement
+  virtual void set_learned(const MlpDev* w, int decide_every, int full_every, float threshold) = 0;
+  virtual void set_trace(unsigned short* x, uint8_t* chg, int cap, int n0) = 0;
+  virtual int trace_count() const = 0;
+  virtual void learn_counters(long long* out) = 0;
 };
 
 template <typename real>
@@ -189,6 +195,23 @@ struct Ctx : CtxBase {
   int n_src = 0;
   bool issued_first = false;
   std::vector<void*> allocs;
+  // learned refinement (MWDG2, eps > 0, one MRA band of level-6 units)
+  bool learn = false;
+  MlpDev lw{};
+  int l_decide = 8, l_full = 32;
+  float l_thr = 0.5f;
+  long long lstep = 0, l_steps = 0;
+  bool l_need = true;
+  unsigned short* tr_x = nullptr;             // trace recording: features at each step's start
+  uint8_t* tr_chg = nullptr;                  // ... and the step's per-unit tree change
+  int tr_cap = 0, tr_n = 0;
+  uint8_t *umode = nullptr, *uchg = nullptr;
+  int* uage = nullptr;
+  float* ustat = nullptr;
+  unsigned short* ux = nullptr;
+  unsigned long long* lcnt = nullptr;         // [2][CNT_SLOTS]: frozen unit-steps, parents kept (not reconstructed)
+  unsigned short* lwb = nullptr;              // the network's BF16 weights (a copy)
+  float* lwf = nullptr;                       // ... and its biases
 
   template <typename T>
   T* alloc(size_t n) {
@@ -548,7 +571,18 @@ struct Ctx : CtxBase {
   void encode_launch() {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
+    if (lrn()) {
+      timed("encode", [&] {
+        if (l_stats())
+          launch_k((k_encode_band_learn<real, true>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                   xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat);
+        else
+          launch_k((k_encode_band_learn<real, false>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev,
+                   nw, xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat);
+      });
+    }
     for (const auto& b : bands()) {
+      if (lrn()) break;
       const int nwg = 1 << (2 * b.first);
       timed("encode", [&] {
         if (mwm)
@@ -579,6 +613,15 @@ struct Ctx : CtxBase {
         launch_k((k_topdown_top<real, false>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
     });
+    if (lrn()) {
+      timed("decode", [&] {
+        launch_k((k_topdown_band_learn<real>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
+                 (const uint8_t*)xflag, leaflev, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[1],
+                 (const uint8_t*)umode, lcnt, uchg, uage,
+                 tr_x ? tr_chg + (size_t)tr_n * N_UNITS : (uint8_t*)nullptr);
+      });
+      return;
+    }
     const auto bs = bands();
     for (auto it = bs.rbegin(); it != bs.rend(); ++it) {
       const int nwg = 1 << (2 * it->first);
@@ -697,6 +740,113 @@ struct Ctx : CtxBase {
     });
   }
 
+  // ---- learned refinement ----
+  bool lrn() const { return (learn || tr_x) && mwm && hc.epsilon > real(0); }
+
+  void learn_alloc() {
+    if (!mwm || !(hc.epsilon > real(0)))
+      throw std::invalid_argument("learned refinement needs the MWDG2 solver and epsilon > 0");
+    const auto b = bands();
+    if (b.size() != 1 || b[0].first != UNIT_LEVEL || b[0].second != L - 1)
+      throw std::invalid_argument("learned refinement needs 7 <= max_ref_lvl <= 12 (one MRA band of level-6 units)");
+    if (umode) return;
+    umode = alloc<uint8_t>(N_UNITS);
+    uchg = alloc<uint8_t>(N_UNITS);
+    uage = alloc<int>(N_UNITS);
+    ustat = alloc<float>((size_t)N_UNITS * USTAT);
+    ux = alloc<unsigned short>((size_t)N_UNITS * NF);
+    lcnt = alloc<unsigned long long>(2 * (size_t)CNT_SLOTS);
+    MW_HIP(hipMemsetAsync(umode, 1, N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(uchg, 0, N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(uage, 0, sizeof(int) * N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(ustat, 0, sizeof(float) * N_UNITS * USTAT, stream));
+    MW_HIP(hipMemsetAsync(lcnt, 0, sizeof(unsigned long long) * 2 * CNT_SLOTS, stream));
+  }
+  void features_launch(unsigned short* out) {
+    launch_k((k_unit_features<real>), dim3(nblk(N_UNITS)), dim3(WG), 0, stream, (const DevState<real>*)st,
+             hier[main_h], zh, (const float*)ustat, (const uint8_t*)uchg, (const int*)uage, L, out);
+  }
+  // At the start of a step: the unit modes for this step. Trace recording
+  // runs the plain method (every unit watched) and records the features.
+  // Otherwise every l_full-th step is plain, and the step after it and every
+  // l_decide-th step the network re-decides (units that changed in the last
+  // step stay watched).
+  void learn_decide() {
+    timed("learn", [&] {
+      if (tr_x) {
+        if (tr_n >= tr_cap) throw std::invalid_argument("trace buffer full");
+        launch_k(k_fill_u8, dim3(nblk(N_UNITS)), dim3(WG), 0, stream, umode, N_UNITS, (uint8_t)1);
+        features_launch(tr_x + (size_t)tr_n * N_UNITS * NF);
+      } else if (lstep % l_full == 0) {
+        launch_k(k_fill_u8, dim3(nblk(N_UNITS)), dim3(WG), 0, stream, umode, N_UNITS, (uint8_t)1);
+        l_need = true;
+      } else if (l_need || lstep % l_decide == 0) {
+        features_launch(ux);
+        launch_k(k_mlp_forward, dim3(N_UNITS / MLP_BLOCK), dim3(64 * MLP_WAVES), 0, stream, lw,
+                 (const unsigned short*)ux, N_UNITS, (float*)nullptr, umode, (const uint8_t*)uchg, l_thr);
+        l_need = false;
+      }
+    });
+  }
+  // the unit statistics are needed by the features of the next step's
+  // decision (or of every step of a trace)
+  bool l_stats() const {
+    if (tr_x) return true;
+    const long long nx = lstep + 1;
+    return (nx % l_full != 0) && (l_need || nx % l_decide == 0);
+  }
+  // after the rebuild (the decode band recorded each unit's tree change)
+  void learn_after() {
+    if (tr_x) ++tr_n;
+    ++lstep;
+    ++l_steps;
+  }
+  void set_learned(const MlpDev* w, int decide_every, int full_every, float threshold) override {
+    if (!w) { learn = false; return; }
+    if (decide_every < 1 || full_every < 1) throw std::invalid_argument("decide_every and full_every must be >= 1");
+    learn_alloc();
+    // the context keeps its own copy of the network
+    if (!lwb) {
+      lwb = alloc<unsigned short>(NH * NF + 2 * NH * NH + NH);
+      lwf = alloc<float>(2 * NH + 1);
+    }
+    unsigned short* b = lwb;
+    const std::pair<const void*, size_t> parts[7] = {{w->w1, 2 * NH * NF}, {w->w2, 2 * NH * NH}, {w->w2t, 2 * NH * NH},
+                                                     {w->w3, 2 * NH}, {w->b1, 4 * NH}, {w->b2, 4 * NH}, {w->b3, 4}};
+    char* dst[7] = {(char*)b, (char*)(b + NH * NF), (char*)(b + NH * NF + NH * NH), (char*)(b + NH * NF + 2 * NH * NH),
+                    (char*)lwf, (char*)(lwf + NH), (char*)(lwf + 2 * NH)};
+    for (int k = 0; k < 7; ++k)
+      MW_HIP(hipMemcpyAsync(dst[k], parts[k].first, parts[k].second, hipMemcpyDeviceToDevice, stream));
+    lw = {b, b + NH * NF, b + NH * NF + NH * NH, b + NH * NF + 2 * NH * NH, lwf, lwf + NH, lwf + 2 * NH};
+    l_decide = decide_every;
+    l_full = full_every;
+    l_thr = threshold;
+    lstep = 0;
+    l_need = true;
+    learn = true;
+  }
+  void set_trace(unsigned short* x, uint8_t* chg, int cap, int n0) override {
+    if (!x) { tr_x = nullptr; tr_chg = nullptr; return; }
+    learn_alloc();
+    tr_x = x;
+    tr_chg = chg;
+    tr_cap = cap;
+    tr_n = n0;
+  }
+  int trace_count() const override { return tr_n; }
+  void learn_counters(long long* out) override {
+    out[0] = l_steps;
+    out[1] = out[2] = 0;
+    if (!lcnt) return;
+    std::vector<unsigned long long> v(2 * (size_t)CNT_SLOTS);
+    MW_HIP(hipMemcpyAsync(v.data(), lcnt, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    for (int k = 0; k < CNT_SLOTS; ++k) {
+      out[1] += (long long)v[k];
+      out[2] += (long long)v[CNT_SLOTS + k];
+    }
+  }
+
   void rebuild(bool sweep = false) {
     encode_launch();
     decode_launch();
@@ -708,7 +858,11 @@ struct Ctx : CtxBase {
     k_prologue_launch();
     k_sources_launch();
     const bool rb = hc.epsilon > real(0.0) || !issued_first;
-    if (rb) rebuild(fuse_sweep1());
+    if (rb) {
+      if (lrn()) learn_decide();
+      rebuild(fuse_sweep1());
+      if (lrn()) learn_after();
+    }
     stage1_launch(rb && fuse_sweep1());
     if (mwm) {
       if (hc.epsilon > real(0.0)) encode_scale_launch();
@@ -1076,7 +1230,15 @@ struct Ctx : CtxBase {
       dev = (name == "leaves") ? (void*)leaf : (void*)nbr[name[3] - '0'];
     } else if (name == "leaf_dt" && leafdt) {
       len = read_state().n_leaves; esz = sizeof(real); dev = leafdt;
-    } else if (name == "rev_z_order") { dev = rev_z; len = N; esz = sizeof(int); }
+    } else if (name == "ustat" && ustat) { dev = ustat; len = (long long)N_UNITS * USTAT; esz = sizeof(float); }
+    else if (name == "uchg" && uchg) { dev = uchg; len = N_UNITS; esz = 1; }
+    else if (name == "uage" && uage) { dev = uage; len = N_UNITS; esz = sizeof(int); }
+    else if (name == "umode" && umode) { dev = umode; len = N_UNITS; esz = 1; }
+    else if (name == "ufeat" && ux && !up) {      // features of the current state, BF16 bits
+      features_launch(ux);
+      dev = ux; len = (long long)N_UNITS * NF; esz = 2;
+    }
+    else if (name == "rev_z_order") { dev = rev_z; len = N; esz = sizeof(int); }
     else if (name == "rev_row_major") { dev = rev_rm; len = N; esz = sizeof(int); }
     if (!dev) throw std::invalid_argument("unknown or absent array " + name);
     if (host && count > 0) {
@@ -1175,6 +1337,259 @@ struct Ctx : CtxBase {
   }
 };
 
+// ----------------------------------------------------------------------------
+// The refinement network: FP32 master parameters with Adam moments, BF16
+// copies for the MFMA kernels, and a device-resident dataset of recorded
+// unit features and tree changes.
+// ----------------------------------------------------------------------------
+struct Mlp {
+  hipStream_t stream = nullptr;
+  float *prm = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+  unsigned short *w1 = nullptr, *w2 = nullptr, *w2t = nullptr, *w3 = nullptr;
+  unsigned short* x = nullptr;      // recorded features, steps x units x 48 (BF16)
+  uint8_t* chg = nullptr;           // recorded tree changes, steps x units
+  uint8_t* y = nullptr;             // labels of the first n_samples rows of x
+  int cap_steps = 0, n_steps = 0;
+  long long n_samples = 0;
+  float* part = nullptr;            // per-wave gradient partials
+  int part_waves = 0;
+  float* lossb = nullptr;           // per-batch mean losses of an epoch
+  long long lossb_n = 0;
+  int adam_t = 0;
+  std::vector<void*> allocs;
+
+  template <typename T>
+  T* alloc(size_t n) {
+    T* p = dalloc<T>(n);
+    allocs.push_back(p);
+    return p;
+  }
+  void release(void* p) {
+    if (!p) return;
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+    (void)hipFree(p);
+  }
+  ~Mlp() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  MlpDev dev() const { return {w1, w2, w2t, w3, prm + P_B1, prm + P_B2, prm + P_B3}; }
+
+  // Glorot-uniform weights from SplitMix64, zero biases
+  void create(unsigned long long seed) {
+    MW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    prm = alloc<float>(N_PARAM);
+    m1 = alloc<float>(N_PARAM);
+    m2 = alloc<float>(N_PARAM);
+    grad = alloc<float>(N_PARAM);
+    w1 = alloc<unsigned short>(NH * NF);
+    w2 = alloc<unsigned short>(NH * NH);
+    w2t = alloc<unsigned short>(NH * NH);
+    w3 = alloc<unsigned short>(NH);
+    std::vector<float> h(N_PARAM, 0.f);
+    unsigned long long z = seed;
+    auto next = [&]() {
+      z += 0x9E3779B97F4A7C15ull;
+      unsigned long long r = z;
+      r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+      r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+      r ^= r >> 31;
+      return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int n, int fan_in, int fan_out) {
+      const double a = std::sqrt(6.0 / (fan_in + fan_out));
+      for (int i = 0; i < n; ++i) h[off + i] = (float)((2.0 * next() - 1.0) * a);
+    };
+    fill(P_W1, NH * NF, NF, NH);
+    fill(P_W2, NH * NH, NH, NH);
+    fill(P_W3, NH, NH, 1);
+    set_params(h.data());
+  }
+  void set_params(const float* h) {
+    MW_HIP(hipMemcpyAsync(prm, h, sizeof(float) * N_PARAM, hipMemcpyHostToDevice, stream));
+    MW_HIP(hipMemsetAsync(m1, 0, sizeof(float) * N_PARAM, stream));
+    MW_HIP(hipMemsetAsync(m2, 0, sizeof(float) * N_PARAM, stream));
+    hipLaunchKernelGGL(k_mlp_refresh, dim3(nblk(N_PARAM)), dim3(WG), 0, stream, (const float*)prm, w1, w2, w2t, w3);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipStreamSynchronize(stream));
+    adam_t = 0;
+  }
+  void get_params(float* h) {
+    MW_HIP(hipMemcpyAsync(h, prm, sizeof(float) * N_PARAM, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+  }
+  void ensure_part(int waves) {
+    if (waves <= part_waves) return;
+    release(part);
+    part = alloc<float>((size_t)waves * (N_PARAM + 1));
+    part_waves = waves;
+  }
+  static int grad_grid(long long nb) { return (int)std::min<long long>(64, (nb + MLP_BLOCK - 1) / MLP_BLOCK); }
+
+  // host rows -> device BF16
+  unsigned short* upload_x(const float* xh, long long n) {
+    std::vector<unsigned short> b((size_t)n * NF);
+    for (size_t i = 0; i < b.size(); ++i) b[i] = f2bf(xh[i]);
+    unsigned short* d = dalloc<unsigned short>(b.size());
+    MW_HIP(hipMemcpyAsync(d, b.data(), 2 * b.size(), hipMemcpyHostToDevice, stream));
+    return d;
+  }
+  void forward(const float* xh, int n, float* prob) {
+    unsigned short* d = upload_x(xh, n);
+    float* pd = dalloc<float>(n);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(std::max(1, std::min(1024, (n + MLP_BLOCK - 1) / MLP_BLOCK))),
+                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d, n, pd, (uint8_t*)nullptr,
+                       (const uint8_t*)nullptr, 0.f);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(prob, pd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    (void)hipFree(pd);
+    if (e != hipSuccess) throw HipError(std::string("mlp forward: ") + hipGetErrorString(e));
+  }
+  // batch gradient and mean loss of host samples, without an update
+  void gradient(const float* xh, const float* yh, int n, float pos_weight, float* loss, float* g) {
+    unsigned short* d = upload_x(xh, n);
+    std::vector<uint8_t> yb(n);
+    for (int i = 0; i < n; ++i) yb[i] = yh[i] >= 0.5f ? 1 : 0;
+    uint8_t* yd = dalloc<uint8_t>(n);
+    float* ld = dalloc<float>(1);
+    MW_HIP(hipMemcpyAsync(yd, yb.data(), n, hipMemcpyHostToDevice, stream));
+    const int grid = grad_grid(n);
+    ensure_part(grid * MLP_WAVES);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d,
+                       (const uint8_t*)yd, (long long)n, 1LL, 0LL, 0LL, n, pos_weight, part);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(nblk(N_PARAM + 1)), dim3(WG), 0, stream, (const float*)part,
+                       grid * MLP_WAVES, 1.f / (float)n, prm, m1, m2, 0.f, 0, grad, ld, w1, w2, w2t, w3);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(g, grad, sizeof(float) * N_PARAM, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, ld, sizeof(float), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    (void)hipFree(yd);
+    (void)hipFree(ld);
+    if (e != hipSuccess) throw HipError(std::string("mlp gradient: ") + hipGetErrorString(e));
+  }
+  // trace of `steps` plain steps of ctx appended to the dataset
+  void record(CtxBase& c, int steps) {
+    if (steps <= 0) return;
+    const int need = n_steps + steps;
+    if (need > cap_steps) {
+      const int cap = std::max(need, 2 * cap_steps);
+      unsigned short* nx = alloc<unsigned short>((size_t)cap * N_UNITS * NF);
+      uint8_t* nc = alloc<uint8_t>((size_t)cap * N_UNITS);
+      if (n_steps) {
+        MW_HIP(hipMemcpyAsync(nx, x, (size_t)n_steps * N_UNITS * NF * 2, hipMemcpyDeviceToDevice, stream));
+        MW_HIP(hipMemcpyAsync(nc, chg, (size_t)n_steps * N_UNITS, hipMemcpyDeviceToDevice, stream));
+      }
+      MW_HIP(hipStreamSynchronize(stream));
+      release(x);
+      release(chg);
+      x = nx;
+      chg = nc;
+      cap_steps = cap;
+    }
+    c.set_trace(x, chg, cap_steps, n_steps);
+    try {
+      c.step(steps);
+    } catch (...) {
+      c.set_trace(nullptr, nullptr, 0, 0);
+      throw;
+    }
+    MW_HIP(hipStreamSynchronize(c.stream));
+    n_steps = c.trace_count();
+    c.set_trace(nullptr, nullptr, 0, 0);
+  }
+  long long build_labels(int horizon) {
+    if (horizon < 1 || n_steps < horizon) throw std::invalid_argument("not enough recorded steps for the horizon");
+    const long long n = (long long)(n_steps - horizon + 1) * N_UNITS;
+    release(y);
+    y = alloc<uint8_t>(n);
+    hipLaunchKernelGGL(k_unit_labels, dim3(nblk(n)), dim3(WG), 0, stream, (const uint8_t*)chg, n_steps, horizon, y);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipStreamSynchronize(stream));
+    n_samples = n;
+    return n;
+  }
+  // Adam over the dataset, one affine permutation of the samples per epoch
+  void fit(int epochs, int batch, float lr, float pos_weight, float* epoch_loss) {
+    if (!y || n_samples <= 0) throw std::invalid_argument("no labelled samples: record and build labels first");
+    if (batch < MLP_BLOCK) throw std::invalid_argument("batch must be at least 64 samples");
+    const long long n = n_samples;
+    const long long nbatch = (n + batch - 1) / batch;
+    if (nbatch > lossb_n) {
+      release(lossb);
+      lossb = alloc<float>(nbatch);
+      lossb_n = nbatch;
+    }
+    ensure_part(grad_grid(batch) * MLP_WAVES);
+    static const long long primes[] = {1000003, 998244353, 1000000007, 2147483647, 104729, 7919};
+    for (int e = 0; e < epochs; ++e) {
+      long long a = 1;
+      for (long long pcand : primes) {
+        const long long p = pcand + 2 * e;           // odd, different per epoch
+        long long g = p, h = n;
+        while (h) { const long long t = g % h; g = h; h = t; }
+        if (g == 1) { a = p % n; break; }
+      }
+      if (a == 0) a = 1;
+      const long long c = ((long long)e * 7919 + 13) % n;
+      for (long long b = 0; b < nbatch; ++b) {
+        const long long base = b * batch;
+        const int nb = (int)std::min<long long>(batch, n - base);
+        const int grid = grad_grid(nb);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)x,
+                           (const uint8_t*)y, n, a, c, base, nb, pos_weight, part);
+        ++adam_t;
+        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(nblk(N_PARAM + 1)), dim3(WG), 0, stream, (const float*)part,
+                           grid * MLP_WAVES, 1.f / (float)nb, prm, m1, m2, lr, adam_t, (float*)nullptr, lossb + b,
+                           w1, w2, w2t, w3);
+      }
+      MW_HIP(hipGetLastError());
+      std::vector<float> lb(nbatch);
+      MW_HIP(hipMemcpyAsync(lb.data(), lossb, sizeof(float) * nbatch, hipMemcpyDeviceToHost, stream));
+      MW_HIP(hipStreamSynchronize(stream));
+      double sum = 0;
+      for (float v : lb) sum += v;
+      if (epoch_loss) epoch_loss[e] = (float)(sum / nbatch);
+    }
+  }
+  // confusion counts (tp, fp, fn, tn) of the dataset at a threshold
+  void eval(float thr, long long* out) {
+    if (!y || n_samples <= 0) throw std::invalid_argument("no labelled samples");
+    const long long n = n_samples;
+    float* pd = dalloc<float>(n);
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min<long long>(4096, (n + MLP_BLOCK - 1) / MLP_BLOCK)),
+                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)x, (int)n, pd,
+                       (uint8_t*)nullptr, (const uint8_t*)nullptr, 0.f);
+    std::vector<float> p(n);
+    std::vector<uint8_t> yy(n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(p.data(), pd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(yy.data(), y, n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(pd);
+    if (e != hipSuccess) throw HipError(std::string("mlp eval: ") + hipGetErrorString(e));
+    out[0] = out[1] = out[2] = out[3] = 0;
+    for (long long i = 0; i < n; ++i) {
+      const bool pp = p[i] >= thr, t = yy[i] != 0;
+      out[pp ? (t ? 0 : 1) : (t ? 2 : 3)]++;
+    }
+  }
+  long long dataset(float* xo, uint8_t* yo, long long max) {
+    const long long n = std::min(max, n_samples);
+    if (n <= 0) return n_samples;
+    std::vector<unsigned short> b((size_t)n * NF);
+    MW_HIP(hipMemcpyAsync(b.data(), x, 2 * b.size(), hipMemcpyDeviceToHost, stream));
+    if (yo) MW_HIP(hipMemcpyAsync(yo, y, n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
+    if (xo)
+      for (size_t i = 0; i < b.size(); ++i) xo[i] = bf2f(b[i]);
+    return n_samples;
+  }
+};
+
 }  // namespace mw
 
 // ============================================================================
@@ -1182,6 +1597,9 @@ struct Ctx : CtxBase {
 // ============================================================================
 struct mw_ctx {
   std::unique_ptr<mw::CtxBase> impl;
+};
+struct mw_mlp {
+  mw::Mlp impl;
 };
 
 namespace {
@@ -1256,6 +1674,92 @@ int mw_set_point_sources(mw_ctx* ctx, int n, const mw_point_source* src) {
 int mw_step(mw_ctx* ctx, int n) {
   MW_NEED(ctx && n >= 0);
   return guarded([&] { ctx->impl->step(n); });
+}
+
+int mw_mlp_create(unsigned long long seed, mw_mlp** out) {
+  MW_NEED(out);
+  *out = nullptr;
+  return guarded([&] {
+    auto m = std::make_unique<mw_mlp>();
+    m->impl.create(seed);
+    *out = m.release();
+  });
+}
+
+int mw_mlp_destroy(mw_mlp* m) {
+  delete m;
+  return MW_OK;
+}
+
+int mw_mlp_param_count(void) { return mw::N_PARAM; }
+
+int mw_mlp_get_params(mw_mlp* m, float* out) {
+  MW_NEED(m && out);
+  return guarded([&] { m->impl.get_params(out); });
+}
+
+int mw_mlp_set_params(mw_mlp* m, const float* in) {
+  MW_NEED(m && in);
+  return guarded([&] { m->impl.set_params(in); });
+}
+
+int mw_mlp_forward(mw_mlp* m, const float* x, int n, float* prob) {
+  MW_NEED(m && x && prob && n > 0);
+  return guarded([&] { m->impl.forward(x, n, prob); });
+}
+
+int mw_mlp_gradient(mw_mlp* m, const float* x, const float* y, int n, float pos_weight, float* loss, float* grad) {
+  MW_NEED(m && x && y && loss && grad && n > 0 && pos_weight > 0.f);
+  return guarded([&] { m->impl.gradient(x, y, n, pos_weight, loss, grad); });
+}
+
+int mw_mlp_record(mw_mlp* m, mw_ctx* ctx, int steps) {
+  MW_NEED(m && ctx && steps >= 0);
+  return guarded([&] { m->impl.record(*ctx->impl, steps); });
+}
+
+int mw_mlp_build_labels(mw_mlp* m, int horizon, long long* n_samples) {
+  MW_NEED(m && horizon >= 1);
+  return guarded([&] {
+    const long long n = m->impl.build_labels(horizon);
+    if (n_samples) *n_samples = n;
+  });
+}
+
+int mw_mlp_fit(mw_mlp* m, int epochs, int batch, float lr, float pos_weight, float* epoch_loss) {
+  MW_NEED(m && epochs >= 0 && batch > 0 && lr > 0.f && pos_weight > 0.f);
+  return guarded([&] { m->impl.fit(epochs, batch, lr, pos_weight, epoch_loss); });
+}
+
+int mw_mlp_eval(mw_mlp* m, float threshold, long long* counts) {
+  MW_NEED(m && counts);
+  return guarded([&] { m->impl.eval(threshold, counts); });
+}
+
+int mw_mlp_dataset(mw_mlp* m, float* x, unsigned char* y, long long max_samples, long long* n_samples) {
+  MW_NEED(m && max_samples >= 0);
+  return guarded([&] {
+    const long long n = m->impl.dataset(x, y, max_samples);
+    if (n_samples) *n_samples = n;
+  });
+}
+
+int mw_set_learned(mw_ctx* ctx, mw_mlp* m, int decide_every, int full_every, float threshold) {
+  MW_NEED(ctx);
+  return guarded([&] {
+    if (!m) {
+      ctx->impl->set_learned(nullptr, 0, 0, 0.f);
+      return;
+    }
+    MW_HIP(hipStreamSynchronize(m->impl.stream));
+    const mw::MlpDev w = m->impl.dev();
+    ctx->impl->set_learned(&w, decide_every, full_every, threshold);
+  });
+}
+
+int mw_learn_counters(mw_ctx* ctx, long long* out) {
+  MW_NEED(ctx && out);
+  return guarded([&] { ctx->impl->learn_counters(out); });
 }
 
 int mw_sync(mw_ctx* ctx) {

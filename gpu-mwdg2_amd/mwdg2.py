@@ -90,6 +90,21 @@ def lib():
         L.mw_save_outputs.argtypes = [vp, i]
         L.mw_cumulative_records.argtypes = [vp, ctypes.POINTER(d), i, ctypes.POINTER(i)]
         L.mw_write_cumulative_file.argtypes = [vp, cp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        lp = ctypes.POINTER(ll)
+        L.mw_mlp_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+        L.mw_mlp_destroy.argtypes = [vp]
+        L.mw_mlp_get_params.argtypes = [vp, fp]
+        L.mw_mlp_set_params.argtypes = [vp, fp]
+        L.mw_mlp_forward.argtypes = [vp, fp, i, fp]
+        L.mw_mlp_gradient.argtypes = [vp, fp, fp, i, ctypes.c_float, fp, fp]
+        L.mw_mlp_record.argtypes = [vp, vp, i]
+        L.mw_mlp_build_labels.argtypes = [vp, i, lp]
+        L.mw_mlp_fit.argtypes = [vp, i, i, ctypes.c_float, ctypes.c_float, fp]
+        L.mw_mlp_eval.argtypes = [vp, ctypes.c_float, lp]
+        L.mw_mlp_dataset.argtypes = [vp, fp, vp, ll, lp]
+        L.mw_set_learned.argtypes = [vp, vp, i, i, ctypes.c_float]
+        L.mw_learn_counters.argtypes = [vp, lp]
         _lib = L
     return _lib
 
@@ -99,8 +114,9 @@ def _check(rc: int):
         raise MWError(rc, lib().mw_last_error().decode())
 
 
-_BYTE = ("sig", "sig_next", "xflag", "leaflev", "preflag")
-_INT = ("leaves", "nbr0", "nbr1", "nbr2", "nbr3", "rev_z_order", "rev_row_major")
+_BYTE = ("sig", "sig_next", "xflag", "leaflev", "preflag", "uchg", "umode")
+_INT = ("leaves", "nbr0", "nbr1", "nbr2", "nbr3", "rev_z_order", "rev_row_major", "uage")
+_F32 = ("ustat",)
 
 
 class Simulation:
@@ -155,7 +171,27 @@ class Simulation:
             return np.uint8
         if name in _INT:
             return np.int32
+        if name in _F32:
+            return np.float32
+        if name == "ufeat":
+            return np.uint16
         return self.real
+
+    def unit_features(self) -> np.ndarray:
+        """The 48 features of every unit for the current state (4096 x 48,
+        the BF16 values the network reads)."""
+        b = self.get("ufeat").astype(np.uint32) << 16
+        return b.view(np.float32).reshape(-1, 48)
+
+    def set_learned(self, mlp, decide_every: int = 8, full_every: int = 32, threshold: float = 0.5):
+        """Learned step with a copy of `mlp`'s network (None: plain step)."""
+        _check(lib().mw_set_learned(self._h, mlp._h if mlp is not None else None, int(decide_every),
+                                    int(full_every), float(threshold)))
+
+    def learn_counters(self) -> dict:
+        out = (ctypes.c_longlong * 3)()
+        _check(lib().mw_learn_counters(self._h, out))
+        return {"steps": out[0], "frozen_unit_steps": out[1], "parents_kept": out[2]}
 
     def get(self, name: str) -> np.ndarray:
         n = ctypes.c_longlong()
@@ -256,3 +292,89 @@ def filter_table(precision: int = 8) -> np.ndarray:
 def run_simulation(argv) -> int:
     a = (ctypes.c_char_p * len(argv))(*[x.encode() for x in argv])
     return lib().mw_run_simulation(len(argv), a)
+
+
+N_PARAM = 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The 48 -> 64 -> 64 -> 1 refinement network (BF16 weights on the matrix
+    cores) with its device-resident training set."""
+
+    def __init__(self, seed: int = 1):
+        h = ctypes.c_void_p()
+        _check(lib().mw_mlp_create(int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mw_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(N_PARAM, dtype=np.float32)
+        _check(lib().mw_mlp_get_params(self._h, _fptr(out)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == N_PARAM
+        _check(lib().mw_mlp_set_params(self._h, _fptr(a)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 48)
+        out = np.empty(x.shape[0], dtype=np.float32)
+        _check(lib().mw_mlp_forward(self._h, _fptr(x), x.shape[0], _fptr(out)))
+        return out
+
+    def gradient(self, x, y, pos_weight: float = 1.0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 48)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(N_PARAM, dtype=np.float32)
+        _check(lib().mw_mlp_gradient(self._h, _fptr(x), _fptr(y), x.shape[0], float(pos_weight), _fptr(loss),
+                                     _fptr(g)))
+        return float(loss[0]), g
+
+    def record(self, sim: "Simulation", steps: int):
+        _check(lib().mw_mlp_record(self._h, sim._h, int(steps)))
+
+    def build_labels(self, horizon: int = 8) -> int:
+        n = ctypes.c_longlong()
+        _check(lib().mw_mlp_build_labels(self._h, int(horizon), ctypes.byref(n)))
+        return n.value
+
+    def fit(self, epochs: int, batch: int = 4096, lr: float = 1e-3, pos_weight: float = 1.0) -> np.ndarray:
+        out = np.zeros(max(1, epochs), dtype=np.float32)
+        _check(lib().mw_mlp_fit(self._h, int(epochs), int(batch), float(lr), float(pos_weight), _fptr(out)))
+        return out[:epochs]
+
+    def eval(self, threshold: float = 0.5) -> dict:
+        c = (ctypes.c_longlong * 4)()
+        _check(lib().mw_mlp_eval(self._h, float(threshold), c))
+        return {"tp": c[0], "fp": c[1], "fn": c[2], "tn": c[3]}
+
+    def dataset(self, max_samples: int | None = None):
+        n = ctypes.c_longlong()
+        _check(lib().mw_mlp_dataset(self._h, None, None, 0, ctypes.byref(n)))
+        m = n.value if max_samples is None else min(n.value, max_samples)
+        x = np.empty((m, 48), dtype=np.float32)
+        y = np.empty(m, dtype=np.uint8)
+        if m:
+            _check(lib().mw_mlp_dataset(self._h, _fptr(x), y.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
+        return x, y

@@ -196,6 +196,54 @@ int mw_cumulative_records(mw_ctx* ctx, double* out, int max_rows, int* n_rows);
 /* Writes the rows as the reference's .cumu file; path NULL = dirroot/resroot.cumu. */
 int mw_write_cumulative_file(mw_ctx* ctx, const char* path);
 
+/* ---- Learned refinement ----------------------------------------------------
+ * No reference counterpart: the reference always re-encodes and decodes every
+ * parent of the previous tree (mra/encode_flow.cu:3-92, decoding.cu:3-67).
+ * A unit is a level-6 node with its subtree (4096 units; needs MWDG2,
+ * epsilon > 0 and 7 <= max_ref_lvl <= 12). A 48 -> 64 -> 64 -> 1 network
+ * (BF16 weights, FP32 accumulation on the matrix cores, ReLU, sigmoid output)
+ * predicts from 48 unit features whether a unit's significance tree changes
+ * within the next steps. In the learned step a unit predicted quiet is
+ * frozen: its parents get scale-only re-encodes and its tree and leaves are
+ * kept (no threshold, no decode round trip); every full_every-th step is the
+ * plain step, and units whose tree changed in the last step stay watched.
+ * Parameter vector (mw_mlp_param_count() floats): W1[64][48] b1[64]
+ * W2[64][64] b2[64] W3[64] b3. */
+typedef struct mw_mlp mw_mlp;
+/* A network with Glorot-uniform weights drawn from SplitMix64(seed). */
+int mw_mlp_create(unsigned long long seed, mw_mlp** out);
+int mw_mlp_destroy(mw_mlp* m);
+int mw_mlp_param_count(void);
+int mw_mlp_get_params(mw_mlp* m, float* out);
+/* Uploads the FP32 master parameters (BF16 copies refreshed, Adam reset). */
+int mw_mlp_set_params(mw_mlp* m, const float* in);
+/* Probabilities of n samples x (n x 48 floats, rounded to BF16). */
+int mw_mlp_forward(mw_mlp* m, const float* x, int n, float* prob);
+/* Mean weighted binary cross-entropy of n samples (y 0/1; positives weigh
+ * pos_weight) and its gradient (param_count floats), without an update. */
+int mw_mlp_gradient(mw_mlp* m, const float* x, const float* y, int n, float pos_weight, float* loss, float* grad);
+/* Runs `steps` plain steps of ctx, recording each step's unit features (at
+ * the step's start) and per-unit tree changes into the network's dataset. */
+int mw_mlp_record(mw_mlp* m, mw_ctx* ctx, int steps);
+/* Labels the recorded steps: a sample (step t, unit u) is positive if u's tree
+ * changed in any of the steps t .. t+horizon-1. */
+int mw_mlp_build_labels(mw_mlp* m, int horizon, long long* n_samples);
+/* Adam (lr, beta 0.9/0.999) over the labelled samples in batches; epoch_loss
+ * (optional) receives each epoch's mean batch loss. */
+int mw_mlp_fit(mw_mlp* m, int epochs, int batch, float lr, float pos_weight, float* epoch_loss);
+/* counts[4] = true positives, false positives, false negatives, true
+ * negatives of the labelled samples at threshold. */
+int mw_mlp_eval(mw_mlp* m, float threshold, long long* counts);
+/* Copies up to max_samples labelled samples (x as floats, y 0/1). */
+int mw_mlp_dataset(mw_mlp* m, float* x, unsigned char* y, long long max_samples, long long* n_samples);
+/* Switches ctx to the learned step with a copy of m's network (m NULL: back
+ * to the plain step): a decision every decide_every steps, a plain step
+ * every full_every steps, units with probability >= threshold watched. */
+int mw_set_learned(mw_ctx* ctx, mw_mlp* m, int decide_every, int full_every, float threshold);
+/* out[3] = learned/traced steps, frozen unit-steps, parents of frozen units
+ * that kept their children instead of reconstructing them. */
+int mw_learn_counters(mw_ctx* ctx, long long* out);
+
 /* Drop-in for the reference executable: argv[argc-1] is the .par file
  * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */
 int mw_run_simulation(int argc, const char* const* argv);

@@ -1,0 +1,183 @@
+"""Learned refinement on the GPU.
+
+- The network's forward probabilities, loss and gradient (BF16 operands on
+  the matrix cores, FP32 accumulation) against oracle/mlp_oracle.py with the
+  same BF16 roundings; the oracle's backward is pinned by finite differences
+  (tests/test_learn_oracle.py). Tolerances: probabilities 1e-5, loss 1e-5
+  relative, gradient 2e-3 relative in the L2 norm (FP32 accumulation order
+  and the rare BF16 tie that rounds the other way).
+- Trace recording runs the plain step: the hierarchy, masks and leaves are
+  bit-identical to a run without it.
+- The recorded features and labels against the oracle's restatement from
+  the state downloaded before each step.
+- A trained network: loss falls, it separates the held-out trace better than
+  the untrained one, and the learned step freezes units while staying within
+  1e-3 m of the FP64 plain solution and conserving volume.
+"""
+import numpy as np
+import pytest
+
+import cases
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(x):
+    return mo.bf16(np.asarray(x, dtype=np.float32))
+
+
+@pytest.mark.parametrize("n", [300, 1000])
+def test_mlp_forward_and_gradient_match_oracle(n):
+    import mwdg2
+    rng = np.random.default_rng(n)
+    with mwdg2.Mlp(seed=5) as m:
+        p = m.params.copy()
+        p[mo.P_B1:mo.P_W2] = rng.normal(0, 0.1, mo.NH)
+        p[mo.P_B2:mo.P_W3] = rng.normal(0, 0.1, mo.NH)
+        p[mo.P_B3] = 0.2
+        m.params = p
+        x = _bf(rng.normal(0, 1, (n, mo.NF)))
+        y = (rng.random(n) < 0.3).astype(np.float32)
+        prob = m.forward(x)
+        loss, g = m.gradient(x, y, pos_weight=3.0)
+    f = mo.forward(p, x, rounding=True)
+    lo, go = mo.loss_grad(p, x, y, pos_weight=3.0, rounding=True)
+    assert np.max(np.abs(prob - f["p"])) < 1e-5
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
+    err = np.linalg.norm(g - go) / np.linalg.norm(go)
+    assert err < 2e-3, err
+    for a, b in ((mo.P_W1, mo.P_B1), (mo.P_B1, mo.P_W2), (mo.P_W2, mo.P_B2), (mo.P_B2, mo.P_W3), (mo.P_W3, mo.P_B3)):
+        e = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
+        assert e < 5e-3, (a, b, e)
+    assert abs(g[mo.P_B3] - go[mo.P_B3]) <= 1e-5 * max(1e-3, abs(go[mo.P_B3]))
+
+
+def test_trace_recording_runs_the_plain_step(case_dir):
+    import mwdg2
+    par = cases.urban(str(case_dir / "lrn_plain8"), L=8, t_end=100.0)
+    with mwdg2.Simulation(par, "f32") as a, mwdg2.Simulation(par, "f32") as b, mwdg2.Mlp(1) as m:
+        a.step(12)
+        m.record(b, 12)
+        for name in ("A.eta0", "A.eta1x", "A.qx0", "A.qy1y", "sig", "leaves", "leaflev"):
+            assert np.array_equal(a.get(name), b.get(name)), name
+        assert a.counters()["time"] == b.counters()["time"]
+
+
+def _unit_state(sim, L):
+    lo, hi = ((1 << 12) - 1) // 3 + 3, ((1 << 14) - 1) // 3 + 3      # level-6 nodes
+    names = ["eta0", "eta1x", "eta1y", "qx0", "qx1x", "qx1y", "qy0", "qy1x", "qy1y"]
+    coef = {n: sim.get("A." + n)[lo:hi].copy() for n in names}
+    zc = {n: sim.get(n)[lo:hi].copy() for n in ("z0", "z1x", "z1y")}
+    return {"coef": coef, "zc": zc, "ustat": sim.get("ustat"), "uchg": sim.get("uchg"), "uage": sim.get("uage"),
+            "sig": sim.get("sig")}
+
+
+def _unit_change(sig_a, sig_b, L):
+    out = np.zeros(mo.N_UNITS, dtype=np.uint8)
+    for lev in range(mo.UNIT_LEVEL, L):
+        per = 1 << (2 * (lev - mo.UNIT_LEVEL))
+        s = ((1 << (2 * lev)) - 1) // 3 + 3
+        d = (sig_a[s:s + per * mo.N_UNITS] != sig_b[s:s + per * mo.N_UNITS]).reshape(mo.N_UNITS, per)
+        out |= d.any(axis=1).astype(np.uint8)
+    return out
+
+
+def test_features_and_labels_match_oracle(case_dir):
+    import mwdg2
+    L, T, H = 8, 10, 4
+    par = cases.urban(str(case_dir / "lrn_feat8"), L=L, t_end=100.0)
+    feats, chgs = [], []
+    with mwdg2.Simulation(par, "f32") as s, mwdg2.Mlp(2) as m:
+        m.record(s, 2)                          # unit statistics exist from here on
+        for t in range(T):
+            pre = _unit_state(s, L)
+            m.record(s, 1)
+            mx = s.counters()["maxes"]
+            feats.append(mo.unit_features(pre["coef"], pre["zc"], pre["ustat"], pre["uchg"], pre["uage"], mx, L))
+            chgs.append(_unit_change(pre["sig"], s.get("sig"), L))
+            assert np.array_equal(s.get("uchg"), chgs[-1])
+        n = m.build_labels(H)
+        x, y = m.dataset()
+    assert n == (T + 2 - H + 1) * mo.N_UNITS and x.shape[0] == n
+    x = x.reshape(T + 2 - H + 1, mo.N_UNITS, mo.NF)
+    y = y.reshape(T + 2 - H + 1, mo.N_UNITS)
+    yo = mo.labels(np.array(chgs), H)
+    assert np.array_equal(y[2:], yo)
+    for t in range(T - H + 1):
+        exp = _bf(feats[t])
+        got = x[t + 2]
+        tol = 2.0 ** -7 * np.abs(exp) + 1e-6
+        bad = np.abs(got - exp) > tol
+        assert not bad.any(), (t, np.argwhere(bad)[:5])
+    assert sum(c.sum() for c in chgs) > 0
+
+
+@pytest.fixture(scope="module")
+def trained(tmp_path_factory):
+    import mwdg2
+    d = tmp_path_factory.mktemp("lrn")
+    L, H = 9, 8
+    m = mwdg2.Mlp(seed=7)
+    for seed in (3, 4):
+        par = cases.urban(str(d / f"train{seed}"), L=L, seed=seed, src_seed=30 + seed, t_end=1800.0)
+        with mwdg2.Simulation(par, "f32") as s:
+            m.record(s, 160)
+    n = m.build_labels(H)
+    losses = m.fit(8, batch=4096, lr=2e-3, pos_weight=2.0)
+    held = mwdg2.Mlp(seed=7)
+    untrained = held.params
+    par = cases.urban(str(d / "held"), L=L, seed=5, src_seed=35, t_end=1800.0)
+    with mwdg2.Simulation(par, "f32") as s:
+        held.record(s, 120)
+    held.build_labels(H)
+    yield {"m": m, "held": held, "untrained": untrained, "losses": losses, "n": n, "L": L, "dir": d}
+    m.close()
+    held.close()
+
+
+def _f1(c):
+    p = c["tp"] / max(1, c["tp"] + c["fp"])
+    r = c["tp"] / max(1, c["tp"] + c["fn"])
+    return 2 * p * r / max(1e-12, p + r), p, r
+
+
+def test_training_separates_held_out_changes(trained):
+    losses = trained["losses"]
+    print("epoch losses", losses)
+    assert losses[-1] < 0.8 * losses[0]
+    held = trained["held"]
+    held.params = trained["m"].params
+    ct = held.eval(0.5)
+    held.params = trained["untrained"]
+    cu = held.eval(0.5)
+    ft, fu = _f1(ct)[0], _f1(cu)[0]
+    print("held-out trained", ct, "f1", ft, "untrained", cu, "f1", fu)
+    assert ft > fu + 0.1
+    assert _f1(ct)[2] > 0.8                     # recall: most changing units are watched
+
+
+def test_learned_step_freezes_units_within_tolerance(trained):
+    import mwdg2
+    from bench import finest_fields
+    L = trained["L"]
+    par = cases.urban(str(trained["dir"] / "run"), L=L, seed=6, src_seed=36, t_end=1800.0)
+    steps = 96
+    with mwdg2.Simulation(par, "f64") as ref:
+        ref.step(steps)
+        h_ref, _, _, v_ref = finest_fields(ref, L)
+        t_ref = ref.counters()["time"]
+    with mwdg2.Simulation(par, "f64") as s:
+        fl = ((1 << (2 * L)) - 1) // 3 + 3
+        s.set_learned(trained["m"], decide_every=8, full_every=32, threshold=0.5)
+        s.step(steps)
+        c = s.learn_counters()
+        h, _, _, v = finest_fields(s, L)
+        t = s.counters()["time"]
+        leaves = s.get("leaves")
+    print("learned counters", c, "dh", float(np.max(np.abs(h - h_ref))), "t", t, t_ref, "volume", v, v_ref)
+    assert c["steps"] == steps and c["frozen_unit_steps"] > 0.2 * steps * mo.N_UNITS
+    assert c["parents_kept"] > 0
+    assert np.max(np.abs(h - h_ref)) <= 1e-3
+    lev = np.floor(np.log2(3 * (leaves.astype(np.int64) - 3) + 1) / 2).astype(np.int64)
+    assert int((4 ** (L - lev)).sum()) == 4 ** L
