@@ -167,6 +167,7 @@ def test_learned_step_freezes_units_within_tolerance(trained):
         ref.step(steps)
         h_ref, _, _, v_ref = finest_fields(ref, L)
         t_ref = ref.counters()["time"]
+        leaves_ref = ref.get("leaves")
     with mwdg2.Simulation(par, "f64") as s:
         fl = ((1 << (2 * L)) - 1) // 3 + 3
         s.set_learned(trained["m"], decide_every=8, full_every=32, threshold=0.5)
@@ -179,5 +180,6 @@ def test_learned_step_freezes_units_within_tolerance(trained):
     assert c["steps"] == steps and c["frozen_unit_steps"] > 0.2 * steps * mo.N_UNITS
     assert c["parents_kept"] > 0
     assert np.max(np.abs(h - h_ref)) <= 1e-3
+    assert np.array_equal(leaves, leaves_ref)          # the guard makes every tree change the plain step makes
     lev = np.floor(np.log2(3 * (leaves.astype(np.int64) - 3) + 1) / 2).astype(np.int64)
     assert int((4 ** (L - lev)).sum()) == 4 ** L
