@@ -150,6 +150,15 @@ struct CtxBase {
   virtual void set_trace(unsigned short* x, uint8_t* chg, int cap, int n0) = 0;
   virtual int trace_count() const = 0;
   virtual void learn_counters(long long* out) = 0;
+  // ensembles
+  virtual void set_sweep_groups(int g) = 0;
+  virtual int sweep_groups() const = 0;
+  virtual int default_sweep_groups() const = 0;
+  virtual void capture_step(hipGraph_t* g) = 0;
+  virtual int parity() const = 0;
+  virtual void set_parity(int c) = 0;
+  virtual void check_graphable() const = 0;
+  bool in_ensemble = false;
 };
 
 template <typename real>
@@ -273,24 +282,7 @@ struct Ctx : CtxBase {
       MW_HIP(hipMemcpyAsync(red, init.data(), sizeof(real) * init.size(), hipMemcpyHostToDevice, stream));
       MW_HIP(hipStreamSynchronize(stream));
     }
-    if (mwm && !hc.limitslopes) {
-      const int G = leaf_grid(), per = G / 8;
-      const long long tmax = nblk(N);
-      const int cap_tiles = (int)(((tmax + 7) / 8 + per - 1) / per) + 1;
-      // regions: G sweep groups of cap entries, or one per emission tile
-      // (4 * EMIT_TILE leaves at most) when the sweep runs inside the emission
-      const int nwg = std::max(G, emit_tiles());
-      const size_t raw = std::max((size_t)G * cap_tiles * WG, (size_t)emit_tiles() * 4 * EMIT_TILE);
-      for (int q = 0; q < 2; ++q) {
-        wet[q].cap = cap_tiles * WG;
-        wet[q].raw = alloc<int>(raw);
-        wet[q].cnt = alloc<int>(nwg);
-        wet[q].off = alloc<int>(nwg + 1);
-        wet[q].list = alloc<int>(N);
-        wet[q].sum = &st->wet_sum[q];
-        MW_HIP(hipMemsetAsync(wet[q].off, 0, sizeof(int) * (nwg + 1), stream));
-      }
-    }
+    if (mwm && !hc.limitslopes) alloc_wet_lists();
     for (int q = 0; q < 3; ++q) {
       cnt[q] = alloc<unsigned long long>(CNT_SLOTS);
       MW_HIP(hipMemsetAsync(cnt[q], 0, sizeof(unsigned long long) * CNT_SLOTS, stream));
@@ -661,7 +653,86 @@ struct Ctx : CtxBase {
                          nbr[1], nbr[2], nbr[3], md, hc.xsz, hc.ysz);
     });
   }
-  int leaf_grid() const { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
+  int sweep_g = 0;                            // work-groups of the leaf sweeps (0: the default)
+  int default_sweep_groups() const override { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
+  int leaf_grid() const { return sweep_g ? sweep_g : default_sweep_groups(); }
+  int sweep_groups() const override { return leaf_grid(); }
+  // The sweeps' work-group count (a multiple of 8, 8 .. 4096). The per-group
+  // reduction slots of the last step are folded into the new layout and the
+  // wet-list regions are re-sized; results do not depend on it (exact
+  // min/max reductions, per-leaf updates).
+  void set_sweep_groups(int g) override {
+    if (!mwm || hc.limitslopes) throw std::invalid_argument("sweep groups apply to the MWDG2 sweeps without limiter");
+    if (g < 8 || g > RED_SLOTS || g % 8) throw std::invalid_argument("sweep groups must be a multiple of 8 in [8, 4096]");
+    MW_HIP(hipStreamSynchronize(stream));
+    const int g0 = leaf_grid();
+    std::vector<real> old((size_t)5 * 2 * g0);
+    MW_HIP(hipMemcpy(old.data(), red, sizeof(real) * old.size(), hipMemcpyDeviceToHost));
+    std::vector<real> nw((size_t)5 * 2 * RED_SLOTS, real(0));
+    for (int i = 0; i < 2 * g; ++i) nw[i] = real(__builtin_inff());
+    for (int r = 0; r < 5; ++r) {
+      real v = old[(size_t)r * 2 * g0];
+      for (int i = 1; i < 2 * g0; ++i) {
+        const real x = old[(size_t)r * 2 * g0 + i];
+        v = (r == 0) ? std::min(v, x) : std::max(v, x);
+      }
+      nw[(size_t)r * 2 * g] = v;
+    }
+    MW_HIP(hipMemcpy(red, nw.data(), sizeof(real) * nw.size(), hipMemcpyHostToDevice));
+    sweep_g = g;
+    alloc_wet_lists();
+    MW_HIP(hipStreamSynchronize(stream));
+  }
+  void release(void* p) {
+    if (!p) return;
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+    (void)hipFree(p);
+  }
+  // wet-leaf lists: G sweep groups of cap entries, or one region per emission
+  // tile (4 * EMIT_TILE leaves at most) when the sweep runs inside the emission
+  void alloc_wet_lists() {
+    const int G = leaf_grid(), per = G / 8;
+    const long long tmax = nblk(N);
+    const int cap_tiles = (int)(((tmax + 7) / 8 + per - 1) / per) + 1;
+    const int nwg = std::max(G, emit_tiles());
+    const size_t raw = std::max((size_t)G * cap_tiles * WG, (size_t)emit_tiles() * 4 * EMIT_TILE);
+    for (int q = 0; q < 2; ++q) {
+      release(wet[q].raw);
+      release(wet[q].cnt);
+      release(wet[q].off);
+      wet[q].cap = cap_tiles * WG;
+      wet[q].raw = alloc<int>(raw);
+      wet[q].cnt = alloc<int>(nwg);
+      wet[q].off = alloc<int>(nwg + 1);
+      if (!wet[q].list) wet[q].list = alloc<int>(N);
+      wet[q].sum = &st->wet_sum[q];
+      MW_HIP(hipMemsetAsync(wet[q].off, 0, sizeof(int) * (nwg + 1), stream));
+    }
+  }
+  void check_graphable() const override {
+    if (!mwm || hc.limitslopes) throw std::invalid_argument("ensembles run the MWDG2 step without limiter");
+    if (learn || tr_x) throw std::invalid_argument("ensembles run the plain step (no learned mode or trace)");
+    if (profiling) throw std::invalid_argument("ensembles need kernel profiling off");
+  }
+  // One step recorded into a graph (nothing runs). The step flips the mask
+  // buffers when it rebuilds the tree, so an ensemble records two
+  // consecutive steps and replays them alternately.
+  void capture_step(hipGraph_t* g) override {
+    check_graphable();
+    MW_HIP(hipStreamSynchronize(stream));
+    MW_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      one_step();
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    MW_HIP(hipStreamEndCapture(stream, g));
+  }
+  int parity() const override { return cur; }
+  void set_parity(int c) override { cur = c; }
   // packs the wet list of stage q written by nwg work-groups with regions of cap entries
   void wet_list_launch(int q, int nwg, int cap) {
     WetList W = wet[q];
@@ -1122,6 +1193,7 @@ struct Ctx : CtxBase {
   bool voutput_stage() const override { return hc.voutput_stage; }
 
   void step(int n) override {
+    if (in_ensemble) throw std::invalid_argument("the context belongs to an ensemble: step the ensemble");
     for (int i = 0; i < n; ++i) one_step();
     MW_HIP(hipGetLastError());
     if (profiling) drain_profile();
@@ -1590,6 +1662,143 @@ struct Mlp {
   }
 };
 
+// ----------------------------------------------------------------------------
+// Ensemble: scenarios of any sizes advanced together. Each scenario's step is
+// recorded as two graphs (consecutive steps: the tree rebuild flips the mask
+// buffers), the graphs of all scenarios become child nodes without
+// dependencies of two parent graphs, and one ensemble step is one launch of a
+// parent graph (alternating). Every scenario keeps its own time step on the
+// device. The optional autotuner picks each scenario's sweep work-group count
+// by timing a few steps at each candidate, then the launch shape: the one
+// parent graph on one stream, or each scenario's graph on its own stream.
+// ----------------------------------------------------------------------------
+struct Ensemble {
+  std::vector<CtxBase*> ctx;
+  std::vector<int> par0, par1;                   // host mask parity after even / odd replays
+  std::vector<int> groups;
+  std::vector<double> tune_ms;                   // per scenario: ms per step at the chosen count
+  hipStream_t stream = nullptr;
+  hipGraph_t parent[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<hipGraph_t> child;
+  std::vector<hipGraphExec_t> own;               // [scenario][2]: each scenario's graphs, instantiated
+  int mode = 0;                                  // 0: parent graph, 1: a graph per scenario stream
+  double mode_ms[2] = {0.0, 0.0};
+  long long k = 0;
+
+  ~Ensemble() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    restore();
+    for (int p = 0; p < 2; ++p) {
+      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+      if (parent[p]) (void)hipGraphDestroy(parent[p]);
+    }
+    for (hipGraphExec_t x : own) (void)hipGraphExecDestroy(x);
+    for (hipGraph_t g : child) (void)hipGraphDestroy(g);
+    for (CtxBase* c : ctx) c->in_ensemble = false;
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void restore() {
+    const size_t n = std::min(par0.size(), par1.size());
+    for (size_t i = 0; i < n; ++i) ctx[i]->set_parity((k & 1) ? par1[i] : par0[i]);
+  }
+  static double time_steps(CtxBase& c, int n) {
+    hipEvent_t a, b;
+    MW_HIP(hipEventCreate(&a));
+    MW_HIP(hipEventCreate(&b));
+    MW_HIP(hipEventRecord(a, c.stream));
+    c.step(n);
+    MW_HIP(hipEventRecord(b, c.stream));
+    MW_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    MW_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / n;
+  }
+  void create(const std::vector<CtxBase*>& cs, int autotune) {
+    ctx = cs;
+    for (CtxBase* c : ctx) {
+      if (c->in_ensemble) throw std::invalid_argument("a context is in another ensemble");
+      c->check_graphable();
+    }
+    MW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (CtxBase* c : ctx) {
+      c->step(1);                                   // the first step builds the tree from the initial state
+      int g = c->sweep_groups();
+      double best = 0.0;
+      if (autotune > 0) {
+        const int g0 = c->default_sweep_groups();
+        for (int cand : {g0, g0 / 2, g0 / 4, g0 / 8}) {
+          if (cand < 8 || cand % 8) continue;
+          c->set_sweep_groups(cand);
+          c->step(1);
+          const double ms = time_steps(*c, autotune);
+          if (best == 0.0 || ms < best) { best = ms; g = cand; }
+        }
+        c->set_sweep_groups(g);
+      }
+      groups.push_back(g);
+      tune_ms.push_back(best);
+      MW_HIP(hipStreamSynchronize(c->stream));
+    }
+    for (int p = 0; p < 2; ++p) {
+      MW_HIP(hipGraphCreate(&parent[p], 0));
+    }
+    for (CtxBase* c : ctx) {
+      par0.push_back(c->parity());
+      hipGraph_t g[2];
+      c->capture_step(&g[0]);
+      par1.push_back(c->parity());
+      c->capture_step(&g[1]);
+      c->set_parity(par0.back());
+      for (int p = 0; p < 2; ++p) {
+        child.push_back(g[p]);
+        hipGraphNode_t node;
+        MW_HIP(hipGraphAddChildGraphNode(&node, parent[p], nullptr, 0, g[p]));
+      }
+      c->in_ensemble = true;
+    }
+    for (int p = 0; p < 2; ++p) MW_HIP(hipGraphInstantiate(&exec[p], parent[p], nullptr, nullptr, 0));
+    for (hipGraph_t g : child) {
+      hipGraphExec_t x;
+      MW_HIP(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+      own.push_back(x);
+    }
+    if (autotune > 0) {
+      for (int m = 0; m < 2; ++m) {
+        mode = m;
+        step(1);
+        wait();
+        const auto t0 = std::chrono::steady_clock::now();
+        step(2 * autotune);
+        wait();
+        mode_ms[m] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                     (2 * autotune);
+      }
+      mode = mode_ms[1] < mode_ms[0] ? 1 : 0;
+    }
+  }
+  void step(int n) {
+    for (int i = 0; i < n; ++i) {
+      if (mode == 0) {
+        MW_HIP(hipGraphLaunch(exec[k & 1], stream));
+      } else {
+        for (size_t j = 0; j < ctx.size(); ++j) MW_HIP(hipGraphLaunch(own[2 * j + (k & 1)], ctx[j]->stream));
+      }
+      ++k;
+    }
+  }
+  void wait() {
+    MW_HIP(hipStreamSynchronize(stream));
+    for (CtxBase* c : ctx) MW_HIP(hipStreamSynchronize(c->stream));
+  }
+  void sync() {
+    wait();
+    restore();
+  }
+};
+
 }  // namespace mw
 
 // ============================================================================
@@ -1600,6 +1809,9 @@ struct mw_ctx {
 };
 struct mw_mlp {
   mw::Mlp impl;
+};
+struct mw_ensemble {
+  mw::Ensemble impl;
 };
 
 namespace {
@@ -1760,6 +1972,70 @@ int mw_set_learned(mw_ctx* ctx, mw_mlp* m, int decide_every, int full_every, flo
 int mw_learn_counters(mw_ctx* ctx, long long* out) {
   MW_NEED(ctx && out);
   return guarded([&] { ctx->impl->learn_counters(out); });
+}
+
+int mw_set_sweep_groups(mw_ctx* ctx, int groups) {
+  MW_NEED(ctx);
+  return guarded([&] { ctx->impl->set_sweep_groups(groups); });
+}
+
+int mw_ensemble_create(mw_ctx* const* ctxs, int n, int autotune_steps, mw_ensemble** out) {
+  MW_NEED(ctxs && n > 0 && out && autotune_steps >= 0);
+  *out = nullptr;
+  return guarded([&] {
+    std::vector<mw::CtxBase*> cs;
+    for (int i = 0; i < n; ++i) {
+      if (!ctxs[i]) throw std::invalid_argument("null context");
+      for (mw::CtxBase* c : cs)
+        if (c == ctxs[i]->impl.get()) throw std::invalid_argument("a context is listed twice");
+      cs.push_back(ctxs[i]->impl.get());
+    }
+    auto e = std::make_unique<mw_ensemble>();
+    e->impl.create(cs, autotune_steps);
+    *out = e.release();
+  });
+}
+
+int mw_ensemble_step(mw_ensemble* e, int steps) {
+  MW_NEED(e && steps >= 0);
+  return guarded([&] { e->impl.step(steps); });
+}
+
+int mw_ensemble_sync(mw_ensemble* e) {
+  MW_NEED(e);
+  return guarded([&] { e->impl.sync(); });
+}
+
+int mw_ensemble_groups(mw_ensemble* e, int* groups, double* ms_per_step) {
+  MW_NEED(e);
+  for (size_t i = 0; i < e->impl.groups.size(); ++i) {
+    if (groups) groups[i] = e->impl.groups[i];
+    if (ms_per_step) ms_per_step[i] = e->impl.tune_ms[i];
+  }
+  return MW_OK;
+}
+
+int mw_ensemble_launch_mode(mw_ensemble* e, int* mode, double* ms_per_step) {
+  MW_NEED(e && mode);
+  *mode = e->impl.mode;
+  if (ms_per_step) {
+    ms_per_step[0] = e->impl.mode_ms[0];
+    ms_per_step[1] = e->impl.mode_ms[1];
+  }
+  return MW_OK;
+}
+
+int mw_ensemble_set_launch_mode(mw_ensemble* e, int mode) {
+  MW_NEED(e && (mode == 0 || mode == 1));
+  return guarded([&] {
+    e->impl.wait();
+    e->impl.mode = mode;
+  });
+}
+
+int mw_ensemble_destroy(mw_ensemble* e) {
+  delete e;
+  return MW_OK;
 }
 
 int mw_sync(mw_ctx* ctx) {

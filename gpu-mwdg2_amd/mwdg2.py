@@ -105,6 +105,14 @@ def lib():
         L.mw_mlp_dataset.argtypes = [vp, fp, vp, ll, lp]
         L.mw_set_learned.argtypes = [vp, vp, i, i, ctypes.c_float]
         L.mw_learn_counters.argtypes = [vp, lp]
+        L.mw_set_sweep_groups.argtypes = [vp, i]
+        L.mw_ensemble_create.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(vp)]
+        L.mw_ensemble_step.argtypes = [vp, i]
+        L.mw_ensemble_sync.argtypes = [vp]
+        L.mw_ensemble_groups.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.mw_ensemble_destroy.argtypes = [vp]
+        L.mw_ensemble_launch_mode.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.mw_ensemble_set_launch_mode.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -187,6 +195,9 @@ class Simulation:
         """Learned step with a copy of `mlp`'s network (None: plain step)."""
         _check(lib().mw_set_learned(self._h, mlp._h if mlp is not None else None, int(decide_every),
                                     int(full_every), float(threshold)))
+
+    def set_sweep_groups(self, groups: int):
+        _check(lib().mw_set_sweep_groups(self._h, int(groups)))
 
     def learn_counters(self) -> dict:
         out = (ctypes.c_longlong * 3)()
@@ -378,3 +389,51 @@ class Mlp:
         if m:
             _check(lib().mw_mlp_dataset(self._h, _fptr(x), y.ctypes.data_as(ctypes.c_void_p), m, ctypes.byref(n)))
         return x, y
+
+
+class Ensemble:
+    """Scenarios advanced together, one graph launch per ensemble step. The
+    simulations must outlive the ensemble and are stepped only through it
+    until it is closed; call sync() before reading them."""
+
+    def __init__(self, sims, autotune_steps: int = 0):
+        self.sims = list(sims)
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._h for s in self.sims])
+        h = ctypes.c_void_p()
+        _check(lib().mw_ensemble_create(arr, len(self.sims), int(autotune_steps), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, n: int = 1):
+        _check(lib().mw_ensemble_step(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().mw_ensemble_sync(self._h))
+
+    def groups(self):
+        g = (ctypes.c_int * len(self.sims))()
+        ms = (ctypes.c_double * len(self.sims))()
+        _check(lib().mw_ensemble_groups(self._h, g, ms))
+        return list(g), list(ms)
+
+    def launch_mode(self):
+        m = ctypes.c_int()
+        ms = (ctypes.c_double * 2)()
+        _check(lib().mw_ensemble_launch_mode(self._h, ctypes.byref(m), ms))
+        return m.value, list(ms)
+
+    def set_launch_mode(self, mode: int):
+        _check(lib().mw_ensemble_set_launch_mode(self._h, int(mode)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mw_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -244,6 +244,35 @@ int mw_set_learned(mw_ctx* ctx, mw_mlp* m, int decide_every, int full_every, flo
  * that kept their children instead of reconstructing them. */
 int mw_learn_counters(mw_ctx* ctx, long long* out);
 
+/* ---- Ensembles -------------------------------------------------------------
+ * No reference counterpart (the reference runs one scenario per process).
+ * Scenarios of any sizes (MWDG2 without limiter, plain step) advance
+ * together: each scenario's step is recorded as HIP graphs, all scenarios are
+ * child graphs of one parent graph, and one ensemble step is one graph launch;
+ * every scenario keeps its own time step. mw_ensemble_create runs each
+ * scenario's first step (and, with autotune_steps > 0, times autotune_steps
+ * steps at each candidate sweep work-group count and keeps the fastest, then
+ * the launch shape, see mw_ensemble_launch_mode).
+ * Between create and destroy the contexts are stepped only through the
+ * ensemble; after mw_ensemble_sync their arrays may be read. Destroy the
+ * ensemble before its contexts. */
+typedef struct mw_ensemble mw_ensemble;
+/* Work-groups of the leaf sweeps (multiple of 8, 8..4096); results do not
+ * depend on it. */
+int mw_set_sweep_groups(mw_ctx* ctx, int groups);
+int mw_ensemble_create(mw_ctx* const* ctxs, int n, int autotune_steps, mw_ensemble** out);
+int mw_ensemble_step(mw_ensemble* e, int steps);
+int mw_ensemble_sync(mw_ensemble* e);
+/* The chosen sweep work-group count and its timed ms per step (0 without
+ * autotuning) of each scenario. */
+int mw_ensemble_groups(mw_ensemble* e, int* groups, double* ms_per_step);
+/* Launch shape: 0 = one launch of the parent graph per step, 1 = each
+ * scenario's graph on its own stream. With autotuning, create times both
+ * (ms_per_step[2]) and keeps the faster; set_launch_mode overrides it. */
+int mw_ensemble_launch_mode(mw_ensemble* e, int* mode, double* ms_per_step);
+int mw_ensemble_set_launch_mode(mw_ensemble* e, int mode);
+int mw_ensemble_destroy(mw_ensemble* e);
+
 /* Drop-in for the reference executable: argv[argc-1] is the .par file
  * (MAIN.cu:16-22, run_simulation.cu:3-800); prints the same summary lines. */
 int mw_run_simulation(int argc, const char* const* argv);
