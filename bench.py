@@ -61,12 +61,13 @@ def kernel_bytes(s: int, L: int, leaves: int, wet1: int, wet2: int) -> dict:
     i = 4
     D = ((1 << (2 * L)) - 1) // 3 + 3
     return {
-        # leaf emission (labels of the level L-1 nodes read, leaf + 4 neighbour
-        # indices written) fused with the stage-1 sweep
-        "emit_sweep1": (D >> 2) + leaves * 5 * i + leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * i,
-        "sweep1": leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * i,
+        # leaf emission (labels of the level L-1 nodes read, leaf indices
+        # written; the 4 neighbour indices only for the wet leaves, which the
+        # wet kernel reads) fused with the stage-1 sweep
+        "emit_sweep1": (D >> 2) + leaves * i + leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * 5 * i,
+        "sweep1": leaves * 12 * s + (leaves - wet1) * 9 * s + wet1 * 5 * i,
         "wet1": wet1 * (12 * s + 9 * s),
-        "sweep2": leaves * 12 * s + (leaves - wet2) * (9 * s + 9 * s) + wet2 * i,
+        "sweep2": leaves * 12 * s + (leaves - wet2) * (9 * s + 9 * s) + wet2 * 5 * i,
         "wet2": wet2 * (12 * s + 9 * s + 9 * s),
         # one-pass kernels (HWFV1 and the limiter path)
         "stage1": leaves * (12 * s + 9 * s),

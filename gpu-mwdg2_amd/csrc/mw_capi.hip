@@ -239,7 +239,9 @@ struct Ctx : CtxBase {
 
   Series<real> series() const { return {s_off, s_len, s_t, s_v}; }
   Sources<real> sources() const { return {n_src, src_code, src_type, src_val, src_row}; }
-  LeafLists leaves() const { return {leaf, {nbr[0], nbr[1], nbr[2], nbr[3]}, &st->n_leaves}; }
+  LeafLists leaves() const {
+    return {leaf, {nbr[0], nbr[1], nbr[2], nbr[3]}, &st->n_leaves, leaflev, {nbr[0], nbr[1], nbr[2], nbr[3]}};
+  }
 
   void create(const mw_config& cfg) {
     prec = sizeof(real);
@@ -1299,7 +1301,17 @@ struct Ctx : CtxBase {
       const DevState<real> s = read_state();
       len = s.n_leaves;
       esz = sizeof(int);
-      dev = (name == "leaves") ? (void*)leaf : (void*)nbr[name[3] - '0'];
+      if (name != "leaves") {
+        const int k = name.size() == 4 ? name[3] - '0' : -1;
+        if (k < 0 || k > 3) throw std::invalid_argument("unknown array " + name);
+        // the MWDG2 sweeps derive neighbours from the labels: materialise them
+        if (mwm && !hc.limitslopes && len > 0)
+          launch_k((k_fill_nbrs<real>), dim3(nblk(len)), dim3(WG), 0, stream, P, (const int*)&st->n_leaves,
+                   (const int*)leaf, (const uint8_t*)leaflev, nbr[0], nbr[1], nbr[2], nbr[3]);
+        dev = nbr[k];
+      } else {
+        dev = leaf;
+      }
     } else if (name == "leaf_dt" && leafdt) {
       len = read_state().n_leaves; esz = sizeof(real); dev = leafdt;
     } else if (name == "ustat" && ustat) { dev = ustat; len = (long long)N_UNITS * USTAT; esz = sizeof(float); }
