@@ -562,7 +562,9 @@ struct Ctx : CtxBase {
     }
     return b;
   }
-  void encode_launch() {
+  // FP32 MWDG2 runs the top levels' encode and decode as one kernel
+  bool fuse_top() const { return mwm && sizeof(real) == 4 && top_level() == TOP_LEVEL; }
+  void encode_launch(bool skip_top = false) {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
     if (lrn()) {
@@ -587,6 +589,7 @@ struct Ctx : CtxBase {
                              xflag, preflag, b.first, b.second, L, hc.epsilon, P.eps_extra_factor, cnt[0]);
       });
     }
+    if (skip_top) return;
     timed("encode_top", [&] {
       if (mwm)
         launch_k((k_encode_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
@@ -596,10 +599,18 @@ struct Ctx : CtxBase {
                            preflag, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0]);
     });
   }
-  void decode_launch() {
+  void decode_launch(bool fused_top = false) {
     uint8_t* prev = sig[cur];
     uint8_t* nw = sig[cur ^ 1];
-    timed("decode_top", [&] {
+    if (fused_top) {
+      if constexpr (sizeof(real) == 4) {         // the node values of levels 0..5 fit the LDS in FP32
+        timed("top_mra", [&] {
+          launch_k((k_top_mra<real>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], (const uint8_t*)prev, nw,
+                   xflag, (const uint8_t*)preflag, leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[0],
+                   cnt[1]);
+        });
+      }
+    } else timed("decode_top", [&] {
       if (mwm)
         launch_k((k_topdown_top<real, true>), dim3(1), dim3(TOP_WG), 0, stream, st, hier[main_h], prev, nw, xflag,
                            leaflev, top_level(), L, hc.epsilon, P.eps_extra_factor, cnt[1]);
@@ -921,8 +932,9 @@ struct Ctx : CtxBase {
   }
 
   void rebuild(bool sweep = false) {
-    encode_launch();
-    decode_launch();
+    const bool ft = fuse_top();
+    encode_launch(ft);
+    decode_launch(ft);
     emit_launch(sweep);
     cur ^= 1;
   }
