@@ -63,6 +63,30 @@ def test_argument_errors():
     assert mwdg2.lib().mw_sort_pairs(-1, None, None, None, None) == 3
 
 
+def test_learned_and_ensemble_argument_errors():
+    """The network, learned-step and ensemble entry points refuse bad
+    arguments before touching the device."""
+    import ctypes
+    lib = mwdg2.lib()
+    assert lib.mw_mlp_param_count() == 64 * 48 + 64 + 64 * 64 + 64 + 64 + 1
+    assert lib.mw_mlp_create(1, None) == 3
+    assert lib.mw_mlp_forward(None, None, 1, None) == 3
+    assert lib.mw_mlp_gradient(None, None, None, 1, ctypes.c_float(1.0), None, None) == 3
+    assert lib.mw_mlp_record(None, None, 1) == 3
+    assert lib.mw_mlp_build_labels(None, 0, None) == 3
+    assert lib.mw_mlp_fit(None, 1, 64, ctypes.c_float(1e-3), ctypes.c_float(1.0), None) == 3
+    assert lib.mw_mlp_eval(None, ctypes.c_float(0.5), None) == 3
+    assert lib.mw_set_learned(None, None, 8, 32, ctypes.c_float(0.5)) == 3
+    assert lib.mw_learn_counters(None, None) == 3
+    assert lib.mw_set_sweep_groups(None, 64) == 3
+    h = ctypes.c_void_p()
+    assert lib.mw_ensemble_create(None, 0, 0, ctypes.byref(h)) == 3
+    assert lib.mw_ensemble_step(None, 1) == 3
+    assert lib.mw_ensemble_sync(None) == 3
+    assert lib.mw_ensemble_destroy(None) == 0
+    assert lib.mw_mlp_destroy(None) == 0
+
+
 def _par(tmp_path, name, edit=None, files=None, **kw):
     par = cases.random_case(str(tmp_path / name), L=4, seed=1, **kw)
     if edit:
