@@ -1523,49 +1523,57 @@ struct Mlp {
   }
   static int grad_grid(long long nb) { return (int)std::min<long long>(64, (nb + MLP_BLOCK - 1) / MLP_BLOCK); }
 
+  // a device buffer for one call, freed on every exit path (after the
+  // stream has drained: the destructor synchronises before freeing)
+  template <typename T>
+  struct Tmp {
+    hipStream_t s;
+    T* p;
+    Tmp(hipStream_t st, size_t n) : s(st), p(dalloc<T>(n)) {}
+    ~Tmp() {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(p);
+    }
+    Tmp(const Tmp&) = delete;
+    Tmp& operator=(const Tmp&) = delete;
+  };
   // host rows -> device BF16
-  unsigned short* upload_x(const float* xh, long long n) {
+  void upload_x(const float* xh, long long n, unsigned short* d) {
     std::vector<unsigned short> b((size_t)n * NF);
     for (size_t i = 0; i < b.size(); ++i) b[i] = f2bf(xh[i]);
-    unsigned short* d = dalloc<unsigned short>(b.size());
     MW_HIP(hipMemcpyAsync(d, b.data(), 2 * b.size(), hipMemcpyHostToDevice, stream));
-    return d;
+    MW_HIP(hipStreamSynchronize(stream));
   }
   void forward(const float* xh, int n, float* prob) {
-    unsigned short* d = upload_x(xh, n);
-    float* pd = dalloc<float>(n);
+    Tmp<unsigned short> d(stream, (size_t)n * NF);
+    Tmp<float> pd(stream, n);
+    upload_x(xh, n, d.p);
     hipLaunchKernelGGL(k_mlp_forward, dim3(std::max(1, std::min(1024, (n + MLP_BLOCK - 1) / MLP_BLOCK))),
-                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d, n, pd, (uint8_t*)nullptr,
-                       (const uint8_t*)nullptr, 0.f);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(prob, pd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(d);
-    (void)hipFree(pd);
-    if (e != hipSuccess) throw HipError(std::string("mlp forward: ") + hipGetErrorString(e));
+                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d.p, n, pd.p,
+                       (uint8_t*)nullptr, (const uint8_t*)nullptr, 0.f);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipMemcpyAsync(prob, pd.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
   }
   // batch gradient and mean loss of host samples, without an update
   void gradient(const float* xh, const float* yh, int n, float pos_weight, float* loss, float* g) {
-    unsigned short* d = upload_x(xh, n);
+    Tmp<unsigned short> d(stream, (size_t)n * NF);
+    Tmp<uint8_t> yd(stream, n);
+    Tmp<float> ld(stream, 1);
+    upload_x(xh, n, d.p);
     std::vector<uint8_t> yb(n);
     for (int i = 0; i < n; ++i) yb[i] = yh[i] >= 0.5f ? 1 : 0;
-    uint8_t* yd = dalloc<uint8_t>(n);
-    float* ld = dalloc<float>(1);
-    MW_HIP(hipMemcpyAsync(yd, yb.data(), n, hipMemcpyHostToDevice, stream));
+    MW_HIP(hipMemcpyAsync(yd.p, yb.data(), n, hipMemcpyHostToDevice, stream));
     const int grid = grad_grid(n);
     ensure_part(grid * MLP_WAVES);
-    hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d,
-                       (const uint8_t*)yd, (long long)n, 1LL, 0LL, 0LL, n, pos_weight, part);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)d.p,
+                       (const uint8_t*)yd.p, (long long)n, 1LL, 0LL, 0LL, n, pos_weight, part);
     hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(nblk(N_PARAM + 1)), dim3(WG), 0, stream, (const float*)part,
-                       grid * MLP_WAVES, 1.f / (float)n, prm, m1, m2, 0.f, 0, grad, ld, w1, w2, w2t, w3);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(g, grad, sizeof(float) * N_PARAM, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(loss, ld, sizeof(float), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(d);
-    (void)hipFree(yd);
-    (void)hipFree(ld);
-    if (e != hipSuccess) throw HipError(std::string("mlp gradient: ") + hipGetErrorString(e));
+                       grid * MLP_WAVES, 1.f / (float)n, prm, m1, m2, 0.f, 0, grad, ld.p, w1, w2, w2t, w3);
+    MW_HIP(hipGetLastError());
+    MW_HIP(hipMemcpyAsync(g, grad, sizeof(float) * N_PARAM, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipMemcpyAsync(loss, ld.p, sizeof(float), hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
   }
   // trace of `steps` plain steps of ctx appended to the dataset
   void record(CtxBase& c, int steps) {
@@ -1654,19 +1662,18 @@ struct Mlp {
   // confusion counts (tp, fp, fn, tn) of the dataset at a threshold
   void eval(float thr, long long* out) {
     if (!y || n_samples <= 0) throw std::invalid_argument("no labelled samples");
+    if (n_samples > INT32_MAX) throw std::invalid_argument("more than 2^31 samples");
     const long long n = n_samples;
-    float* pd = dalloc<float>(n);
+    Tmp<float> pd(stream, n);
     hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)std::min<long long>(4096, (n + MLP_BLOCK - 1) / MLP_BLOCK)),
-                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)x, (int)n, pd,
+                       dim3(64 * MLP_WAVES), 0, stream, dev(), (const unsigned short*)x, (int)n, pd.p,
                        (uint8_t*)nullptr, (const uint8_t*)nullptr, 0.f);
+    MW_HIP(hipGetLastError());
     std::vector<float> p(n);
     std::vector<uint8_t> yy(n);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(p.data(), pd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(yy.data(), y, n, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(pd);
-    if (e != hipSuccess) throw HipError(std::string("mlp eval: ") + hipGetErrorString(e));
+    MW_HIP(hipMemcpyAsync(p.data(), pd.p, sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipMemcpyAsync(yy.data(), y, n, hipMemcpyDeviceToHost, stream));
+    MW_HIP(hipStreamSynchronize(stream));
     out[0] = out[1] = out[2] = out[3] = 0;
     for (long long i = 0; i < n; ++i) {
       const bool pp = p[i] >= thr, t = yy[i] != 0;
@@ -1727,17 +1734,17 @@ struct Ensemble {
     for (size_t i = 0; i < n; ++i) ctx[i]->set_parity((k & 1) ? par1[i] : par0[i]);
   }
   static double time_steps(CtxBase& c, int n) {
-    hipEvent_t a, b;
-    MW_HIP(hipEventCreate(&a));
-    MW_HIP(hipEventCreate(&b));
-    MW_HIP(hipEventRecord(a, c.stream));
+    struct Ev {
+      hipEvent_t e = nullptr;
+      Ev() { MW_HIP(hipEventCreate(&e)); }
+      ~Ev() { (void)hipEventDestroy(e); }
+    } a, b;
+    MW_HIP(hipEventRecord(a.e, c.stream));
     c.step(n);
-    MW_HIP(hipEventRecord(b, c.stream));
-    MW_HIP(hipEventSynchronize(b));
+    MW_HIP(hipEventRecord(b.e, c.stream));
+    MW_HIP(hipEventSynchronize(b.e));
     float ms = 0.f;
-    MW_HIP(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    MW_HIP(hipEventElapsedTime(&ms, a.e, b.e));
     return ms / n;
   }
   void create(const std::vector<CtxBase*>& cs, int autotune) {
