@@ -219,6 +219,8 @@ struct Ctx : CtxBase {
   float* ustat = nullptr;
   unsigned short* ux = nullptr;
   unsigned long long* lcnt = nullptr;         // [2][CNT_SLOTS]: frozen unit-steps, parents kept (not reconstructed)
+  uint8_t* dirty = nullptr;                   // parents whose children changed since their last encode
+  real* dm = nullptr;                         // [3][D] detail maxima of each parent's last encode
   unsigned short* lwb = nullptr;              // the network's BF16 weights (a copy)
   float* lwf = nullptr;                       // ... and its biases
 
@@ -323,6 +325,7 @@ struct Ctx : CtxBase {
 
   void fill_params() {
     P.L = L; P.N = N; P.D = D; P.H = H; P.mesh_dim = md; P.xsz = hc.xsz; P.ysz = hc.ysz;
+    P.dirty = learn ? dirty : nullptr;
     P.solver = hc.solver;
     P.eps = hc.epsilon; P.dx = hc.dx; P.g = hc.g;
     P.tol_h = real(1e-3); P.tol_q = real(0.0); P.tol_s = real(1e-9);   // classes/SolverParams.h:23-25
@@ -547,7 +550,8 @@ struct Ctx : CtxBase {
   void k_sources_launch() {
     if (!n_src) return;
     timed("point_sources", [&] {
-      launch_k((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx);
+      launch_k((k_point_sources<real>), dim3(nblk(n_src)), dim3(WG), 0, stream, st, hier[main_h], sources(), L, hc.dx,
+               P.dirty);
     });
   }
   int top_level() const { return std::min(TOP_LEVEL, L - 1); }
@@ -569,12 +573,15 @@ struct Ctx : CtxBase {
     uint8_t* nw = sig[cur ^ 1];
     if (lrn()) {
       timed("encode", [&] {
+        const int full = (tr_x || !P.dirty || lstep % l_full == 0) ? 1 : 0;
         if (l_stats())
           launch_k((k_encode_band_learn<real, true>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
-                   xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat);
+                   xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat, P.dirty, dm,
+                   D, full);
         else
           launch_k((k_encode_band_learn<real, false>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev,
-                   nw, xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat);
+                   nw, xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat, P.dirty,
+                   dm, D, full);
       });
     }
     for (const auto& b : bands()) {
@@ -623,7 +630,7 @@ struct Ctx : CtxBase {
         launch_k((k_topdown_band_learn<real>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                  (const uint8_t*)xflag, leaflev, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[1],
                  (const uint8_t*)umode, lcnt, uchg, uage,
-                 tr_x ? tr_chg + (size_t)tr_n * N_UNITS : (uint8_t*)nullptr);
+                 tr_x ? tr_chg + (size_t)tr_n * N_UNITS : (uint8_t*)nullptr, P.dirty);
       });
       return;
     }
@@ -840,6 +847,8 @@ struct Ctx : CtxBase {
     ustat = alloc<float>((size_t)N_UNITS * USTAT);
     ux = alloc<unsigned short>((size_t)N_UNITS * NF);
     lcnt = alloc<unsigned long long>(2 * (size_t)CNT_SLOTS);
+    dirty = alloc<uint8_t>(D + 4);
+    dm = alloc<real>((size_t)3 * D);
     MW_HIP(hipMemsetAsync(umode, 1, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uchg, 0, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uage, 0, sizeof(int) * N_UNITS, stream));
@@ -886,7 +895,11 @@ struct Ctx : CtxBase {
     ++l_steps;
   }
   void set_learned(const MlpDev* w, int decide_every, int full_every, float threshold) override {
-    if (!w) { learn = false; return; }
+    if (!w) {
+      learn = false;
+      P.dirty = nullptr;
+      return;
+    }
     if (decide_every < 1 || full_every < 1) throw std::invalid_argument("decide_every and full_every must be >= 1");
     learn_alloc();
     // the context keeps its own copy of the network
@@ -908,6 +921,8 @@ struct Ctx : CtxBase {
     lstep = 0;
     l_need = true;
     learn = true;
+    MW_HIP(hipMemsetAsync(dirty, 1, D + 4, stream));      // the first learned step re-encodes everything
+    P.dirty = dirty;
   }
   void set_trace(unsigned short* x, uint8_t* chg, int cap, int n0) override {
     if (!x) { tr_x = nullptr; tr_chg = nullptr; return; }

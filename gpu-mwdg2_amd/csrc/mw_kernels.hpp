@@ -55,6 +55,7 @@ struct Params {
   int oregon;                     // west HFIX/HVAR is the Oregon Seaside wavemaker (if not monai)
   real massint;                   // stage-output interval
   real* leaf_dt;                  // per-leaf CFL time step of the last update (VTK output), or null
+  uint8_t* dirty;                 // learned step: parents whose children changed since their last encode, or null
   int test_case;                  // 0: raster case; else the built-in synthetic case
   int flow;                       // synthetic cases: FLOW_EW, FLOW_NS or FLOW_NONE
 };
