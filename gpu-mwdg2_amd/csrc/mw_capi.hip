@@ -220,6 +220,9 @@ struct Ctx : CtxBase {
   unsigned short* ux = nullptr;
   unsigned long long* lcnt = nullptr;         // [2][CNT_SLOTS]: frozen unit-steps, parents kept (not reconstructed)
   uint8_t* dirty = nullptr;                   // parents whose children changed since their last encode
+  uint8_t *uquiet = nullptr, *uwet = nullptr, *utouch = nullptr, *uborder = nullptr;
+  bool quiet_off_pending = false;
+  real* umx = nullptr;                        // [unit][4] maxima of a unit's leaves when last swept
   real* dm = nullptr;                         // [3][D] detail maxima of each parent's last encode
   unsigned short* lwb = nullptr;              // the network's BF16 weights (a copy)
   float* lwf = nullptr;                       // ... and its biases
@@ -325,7 +328,7 @@ struct Ctx : CtxBase {
 
   void fill_params() {
     P.L = L; P.N = N; P.D = D; P.H = H; P.mesh_dim = md; P.xsz = hc.xsz; P.ysz = hc.ysz;
-    P.dirty = learn ? dirty : nullptr;
+    if (umode) set_quiet_params();
     P.solver = hc.solver;
     P.eps = hc.epsilon; P.dx = hc.dx; P.g = hc.g;
     P.tol_h = real(1e-3); P.tol_q = real(0.0); P.tol_s = real(1e-9);   // classes/SolverParams.h:23-25
@@ -573,15 +576,16 @@ struct Ctx : CtxBase {
     uint8_t* nw = sig[cur ^ 1];
     if (lrn()) {
       timed("encode", [&] {
-        const int full = (tr_x || !P.dirty || lstep % l_full == 0) ? 1 : 0;
+        const int full = l_full_step() ? 1 : 0;
+        uint8_t* ut = P.uquiet ? utouch : nullptr;
         if (l_stats())
           launch_k((k_encode_band_learn<real, true>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                    xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat, P.dirty, dm,
-                   D, full);
+                   D, full, ut);
         else
           launch_k((k_encode_band_learn<real, false>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev,
                    nw, xflag, preflag, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[0], ustat, P.dirty,
-                   dm, D, full);
+                   dm, D, full, ut);
       });
     }
     for (const auto& b : bands()) {
@@ -630,7 +634,8 @@ struct Ctx : CtxBase {
         launch_k((k_topdown_band_learn<real>), dim3(N_UNITS), dim3(WG), 0, stream, st, hier[main_h], prev, nw,
                  (const uint8_t*)xflag, leaflev, UNIT_LEVEL, L - 1, L, hc.epsilon, P.eps_extra_factor, cnt[1],
                  (const uint8_t*)umode, lcnt, uchg, uage,
-                 tr_x ? tr_chg + (size_t)tr_n * N_UNITS : (uint8_t*)nullptr, P.dirty);
+                 tr_x ? tr_chg + (size_t)tr_n * N_UNITS : (uint8_t*)nullptr, P.dirty,
+                 P.uquiet ? utouch : (uint8_t*)nullptr);
       });
       return;
     }
@@ -846,14 +851,29 @@ struct Ctx : CtxBase {
     uage = alloc<int>(N_UNITS);
     ustat = alloc<float>((size_t)N_UNITS * USTAT);
     ux = alloc<unsigned short>((size_t)N_UNITS * NF);
-    lcnt = alloc<unsigned long long>(2 * (size_t)CNT_SLOTS);
+    lcnt = alloc<unsigned long long>(3 * (size_t)CNT_SLOTS);
     dirty = alloc<uint8_t>(D + 4);
     dm = alloc<real>((size_t)3 * D);
+    uquiet = alloc<uint8_t>(N_UNITS);
+    uwet = alloc<uint8_t>(N_UNITS);
+    utouch = alloc<uint8_t>(N_UNITS);
+    uborder = alloc<uint8_t>(N_UNITS);
+    umx = alloc<real>((size_t)4 * N_UNITS);
+    {
+      // units touching the domain border or reaching past the DEM
+      std::vector<uint8_t> b(N_UNITS);
+      const int side = 1 << (L - UNIT_LEVEL);
+      for (int u = 0; u < N_UNITS; ++u) {
+        const int x0 = compact(u) * side, y0 = compact(u >> 1) * side;
+        b[u] = (x0 == 0 || y0 == 0 || x0 + side - 1 >= hc.xsz - 1 || y0 + side - 1 >= hc.ysz - 1) ? 1 : 0;
+      }
+      MW_HIP(hipMemcpy(uborder, b.data(), N_UNITS, hipMemcpyHostToDevice));
+    }
     MW_HIP(hipMemsetAsync(umode, 1, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uchg, 0, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uage, 0, sizeof(int) * N_UNITS, stream));
     MW_HIP(hipMemsetAsync(ustat, 0, sizeof(float) * N_UNITS * USTAT, stream));
-    MW_HIP(hipMemsetAsync(lcnt, 0, sizeof(unsigned long long) * 2 * CNT_SLOTS, stream));
+    MW_HIP(hipMemsetAsync(lcnt, 0, sizeof(unsigned long long) * 3 * CNT_SLOTS, stream));
   }
   void features_launch(unsigned short* out) {
     launch_k((k_unit_features<real>), dim3(nblk(N_UNITS)), dim3(WG), 0, stream, (const DevState<real>*)st,
@@ -896,8 +916,12 @@ struct Ctx : CtxBase {
   }
   void set_learned(const MlpDev* w, int decide_every, int full_every, float threshold) override {
     if (!w) {
+      // the next prologue still adds the record of the units the last
+      // learned step did not sweep; then the quiet parameters are dropped
       learn = false;
       P.dirty = nullptr;
+      P.uwet = nullptr;
+      quiet_off_pending = P.uquiet != nullptr;
       return;
     }
     if (decide_every < 1 || full_every < 1) throw std::invalid_argument("decide_every and full_every must be >= 1");
@@ -922,7 +946,20 @@ struct Ctx : CtxBase {
     l_need = true;
     learn = true;
     MW_HIP(hipMemsetAsync(dirty, 1, D + 4, stream));      // the first learned step re-encodes everything
-    P.dirty = dirty;
+    quiet_off_pending = false;
+    MW_HIP(hipMemsetAsync(uquiet, 0, N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(uwet, 1, N_UNITS, stream));      // no unit is known dry yet
+    MW_HIP(hipMemsetAsync(utouch, 1, N_UNITS, stream));
+    set_quiet_params();
+  }
+  // quiet units need one emission tile per unit: max_ref_lvl 12
+  void set_quiet_params() {
+    P.dirty = learn ? dirty : nullptr;
+    const bool q = learn && L == 12 && fuse_sweep1() && (1 << (2 * (L - 1 - UNIT_LEVEL))) == EMIT_TILE;
+    P.uquiet = q ? uquiet : nullptr;
+    P.uwet = q ? uwet : nullptr;
+    P.umx = q ? umx : nullptr;
+    P.n_units = N_UNITS;
   }
   void set_trace(unsigned short* x, uint8_t* chg, int cap, int n0) override {
     if (!x) { tr_x = nullptr; tr_chg = nullptr; return; }
@@ -935,27 +972,40 @@ struct Ctx : CtxBase {
   int trace_count() const override { return tr_n; }
   void learn_counters(long long* out) override {
     out[0] = l_steps;
-    out[1] = out[2] = 0;
+    out[1] = out[2] = out[3] = 0;
     if (!lcnt) return;
-    std::vector<unsigned long long> v(2 * (size_t)CNT_SLOTS);
+    std::vector<unsigned long long> v(3 * (size_t)CNT_SLOTS);
     MW_HIP(hipMemcpyAsync(v.data(), lcnt, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost, stream));
     MW_HIP(hipStreamSynchronize(stream));
     for (int k = 0; k < CNT_SLOTS; ++k) {
       out[1] += (long long)v[k];
       out[2] += (long long)v[CNT_SLOTS + k];
+      out[3] += (long long)v[2 * CNT_SLOTS + k];
     }
   }
 
+  bool l_full_step() const { return tr_x || !P.dirty || lstep % l_full == 0; }
   void rebuild(bool sweep = false) {
     const bool ft = fuse_top();
     encode_launch(ft);
     decode_launch(ft);
+    if (lrn() && P.uquiet)
+      timed("learn", [&] {
+        launch_k(k_unit_quiet, dim3(nblk(N_UNITS)), dim3(WG), 0, stream, (const uint8_t*)sig[cur ^ 1],
+                 (const uint8_t*)utouch, (const uint8_t*)uwet, (const uint8_t*)uborder, uquiet,
+                 l_full_step() ? 1 : 0, lcnt + 2 * CNT_SLOTS);
+      });
     emit_launch(sweep);
     cur ^= 1;
   }
 
   void one_step() {
     k_prologue_launch();
+    if (quiet_off_pending) {
+      P.uquiet = nullptr;
+      P.umx = nullptr;
+      quiet_off_pending = false;
+    }
     k_sources_launch();
     const bool rb = hc.epsilon > real(0.0) || !issued_first;
     if (rb) {

@@ -56,6 +56,12 @@ struct Params {
   real massint;                   // stage-output interval
   real* leaf_dt;                  // per-leaf CFL time step of the last update (VTK output), or null
   uint8_t* dirty;                 // learned step: parents whose children changed since their last encode, or null
+  // learned step, max_ref_lvl 12: level-6 units whose two sweeps are skipped
+  // this step (all leaves dry and unchanged, neighbour units dry), or null
+  const uint8_t* uquiet;
+  uint8_t* uwet;                  // units holding a leaf with h >= tol_h after the last stage 2
+  real* umx;                      // [unit][4] maxima of the unit's leaves when it was last swept
+  int n_units;
   int test_case;                  // 0: raster case; else the built-in synthetic case
   int flow;                       // synthetic cases: FLOW_EW, FLOW_NS or FLOW_NONE
 };

@@ -200,9 +200,9 @@ class Simulation:
         _check(lib().mw_set_sweep_groups(self._h, int(groups)))
 
     def learn_counters(self) -> dict:
-        out = (ctypes.c_longlong * 3)()
+        out = (ctypes.c_longlong * 4)()
         _check(lib().mw_learn_counters(self._h, out))
-        return {"steps": out[0], "frozen_unit_steps": out[1], "parents_kept": out[2]}
+        return {"steps": out[0], "frozen_unit_steps": out[1], "parents_kept": out[2], "quiet_unit_steps": out[3]}
 
     def get(self, name: str) -> np.ndarray:
         n = ctypes.c_longlong()

@@ -183,3 +183,29 @@ def test_learned_step_freezes_units_within_tolerance(trained):
     assert np.array_equal(leaves, leaves_ref)          # the guard makes every tree change the plain step makes
     lev = np.floor(np.log2(3 * (leaves.astype(np.int64) - 3) + 1) / 2).astype(np.int64)
     assert int((4 ** (L - lev)).sum()) == 4 ** L
+
+
+def test_quiet_units_full_size(case_dir):
+    """max_ref_lvl 12: dry, unchanged units with dry surroundings skip both
+    sweeps. With every unit frozen (the guard alone) the learned step stays
+    within 1e-3 m of the plain FP64 step over 40 steps (two of them plain)
+    and makes the same tree."""
+    import mwdg2
+    from bench import finest_fields
+    L, steps = 12, 40
+    par = cases.urban(str(case_dir / "quiet12"), L=L, t_end=1800.0)
+    with mwdg2.Simulation(par, "f64") as ref:
+        ref.step(steps)
+        h_ref = finest_fields(ref, L)[0]
+        leaves_ref = ref.get("leaves")
+    with mwdg2.Mlp(3) as m, mwdg2.Simulation(par, "f64") as s:
+        s.set_learned(m, decide_every=8, full_every=32, threshold=2.0)
+        s.step(steps)
+        c = s.learn_counters()
+        h = finest_fields(s, L)[0]
+        leaves = s.get("leaves")
+    dh = float(np.max(np.abs(h - h_ref)))
+    print("quiet counters", c, "dh", dh)
+    assert c["quiet_unit_steps"] > 0.3 * steps * mo.N_UNITS
+    assert dh <= 1e-3
+    assert np.array_equal(leaves, leaves_ref)
