@@ -803,8 +803,10 @@ struct Ctx : CtxBase {
     for (const auto& b : bands()) {
       const int nwg = 1 << (2 * b.first);
       timed("encode_scale", [&] {
+        // quiet units (one band of level-6 units) keep their stage-1 parents
+        const uint8_t* q = (P.uquiet && b.first == UNIT_LEVEL) ? P.uquiet : nullptr;
         launch_k((k_encode_scale_band<real>), dim3(nwg), dim3(WG), 0, stream, st, hier[1], sig[cur], b.first,
-                           b.second, cnt[2]);
+                           b.second, cnt[2], q);
       });
     }
     timed("encode_scale_top", [&] {
