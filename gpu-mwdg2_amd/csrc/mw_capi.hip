@@ -954,10 +954,12 @@ struct Ctx : CtxBase {
     MW_HIP(hipMemsetAsync(utouch, 1, N_UNITS, stream));
     set_quiet_params();
   }
-  // quiet units need one emission tile per unit: max_ref_lvl 12
+  // quiet units: the emission tiles of the fused stage-1 sweep hold whole
+  // units (max_ref_lvl <= 12, EMIT_TILE level-(L-1) items per tile)
   void set_quiet_params() {
     P.dirty = learn ? dirty : nullptr;
-    const bool q = learn && L == 12 && fuse_sweep1() && (1 << (2 * (L - 1 - UNIT_LEVEL))) == EMIT_TILE;
+    static_assert(EMIT_TILE == 1 << (2 * (12 - 1 - UNIT_LEVEL)), "one unit per emission tile at max_ref_lvl 12");
+    const bool q = learn && L <= 12 && fuse_sweep1();
     P.uquiet = q ? uquiet : nullptr;
     P.uwet = q ? uwet : nullptr;
     P.umx = q ? umx : nullptr;

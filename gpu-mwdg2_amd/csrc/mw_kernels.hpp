@@ -10,6 +10,9 @@ constexpr int EMIT_ITEMS = 4;    // level-(L-1) nodes per thread in leaf emissio
 constexpr int EMIT_TILE = WG * EMIT_ITEMS;
 constexpr int MAX_SRC = 4096;
 constexpr int RED_SLOTS = 4096;  // max work-groups of a leaf kernel (one reduction slot each)
+// Units of the learned step: level-6 nodes and their subtrees.
+constexpr int UNIT_LEVEL = 6;
+constexpr int N_UNITS = 1 << (2 * UNIT_LEVEL);
 
 // Scalars that live on the device for the whole run, so that a step needs no
 // host round trip (the reference reads dt, N_a and the maxima back every step).

@@ -242,8 +242,7 @@ int mw_mlp_dataset(mw_mlp* m, float* x, unsigned char* y, long long max_samples,
 int mw_set_learned(mw_ctx* ctx, mw_mlp* m, int decide_every, int full_every, float threshold);
 /* out[4] = learned/traced steps, frozen unit-steps, parents of frozen units
  * that kept their children instead of reconstructing them, unit-steps whose
- * sweeps were skipped (max_ref_lvl 12: dry, unchanged units with dry
- * surroundings). */
+ * sweeps were skipped (dry, unchanged units with dry surroundings). */
 int mw_learn_counters(mw_ctx* ctx, long long* out);
 
 /* ---- Ensembles -------------------------------------------------------------
