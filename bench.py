@@ -10,6 +10,14 @@ input path. A "step" is one full adaptive time step (grid rebuild, both RK
 stages, CFL reduction). Metric: finest-equivalent cell updates per second,
 4^L * steps / wall time of the timed steps (inputs resident in HBM).
 
+Besides the metric (the plain step, bit-exact with the reference method),
+the line carries the dominant kernel's roofline, the whole step's roofline
+(`step_roofline`), the CPU baseline, the learned step (`learned`: the
+network trained on the GPU from plain-step traces of two other layouts,
+held-out precision/recall, step time with the network and with the guard
+alone) and both steps' accuracy against the FP64 product. `--ensemble N`
+instead times N scenarios advanced together (one JSON line).
+
 Prints one JSON line (see the field list at the end of main()).
 """
 from __future__ import annotations
