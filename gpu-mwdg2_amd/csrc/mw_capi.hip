@@ -842,8 +842,8 @@ struct Ctx : CtxBase {
   bool lrn() const { return (learn || tr_x) && mwm && hc.epsilon > real(0); }
 
   void learn_alloc() {
-    if (!mwm || !(hc.epsilon > real(0)))
-      throw std::invalid_argument("learned refinement needs the MWDG2 solver and epsilon > 0");
+    if (!mwm || !(hc.epsilon > real(0)) || hc.limitslopes)
+      throw std::invalid_argument("learned refinement needs the MWDG2 solver without slope limiter and epsilon > 0");
     const auto b = bands();
     if (b.size() != 1 || b[0].first != UNIT_LEVEL || b[0].second != L - 1)
       throw std::invalid_argument("learned refinement needs 7 <= max_ref_lvl <= 12 (one MRA band of level-6 units)");
@@ -953,6 +953,15 @@ struct Ctx : CtxBase {
     MW_HIP(hipMemsetAsync(uwet, 1, N_UNITS, stream));      // no unit is known dry yet
     MW_HIP(hipMemsetAsync(utouch, 1, N_UNITS, stream));
     set_quiet_params();
+  }
+  // After the state changed outside the step (new initial state, uploads):
+  // every parent is re-encoded and no unit is quiet until swept again.
+  void learn_invalidate() {
+    if (!dirty) return;
+    MW_HIP(hipMemsetAsync(dirty, 1, D + 4, stream));
+    MW_HIP(hipMemsetAsync(uquiet, 0, N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(uwet, 1, N_UNITS, stream));
+    MW_HIP(hipMemsetAsync(utouch, 1, N_UNITS, stream));
   }
   // quiet units: the emission tiles of the fused stage-1 sweep hold whole
   // units (max_ref_lvl <= 12, EMIT_TILE level-(L-1) items per tile)
@@ -1363,6 +1372,7 @@ struct Ctx : CtxBase {
       if (host && count > 0) {
         const int n = (int)std::min<long long>(count, len);
         if (up) {
+          learn_invalidate();
           MW_HIP(hipMemcpyAsync(xbuf, host, sizeof(real) * n, hipMemcpyHostToDevice, stream));
           coef_io_launch(hb, coef_id, n, false);
         } else {
@@ -1494,6 +1504,7 @@ struct Ctx : CtxBase {
   }
   void rebuild_all() {
     MW_HIP(hipStreamSynchronize(stream));
+    learn_invalidate();
     clear_hierarchies();
     for (uint8_t* p : {sig[0], sig[1], xflag, leaflev, preflag}) MW_HIP(hipMemsetAsync(p, 0, D + 4, stream));
     fill_params();

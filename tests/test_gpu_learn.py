@@ -209,3 +209,54 @@ def test_quiet_units_full_size(case_dir):
     assert c["quiet_unit_steps"] > 0.3 * steps * mo.N_UNITS
     assert dh <= 1e-3
     assert np.array_equal(leaves, leaves_ref)
+
+
+def _add_water(sim, L, x0, x1, y0, y1, dh):
+    """eta0 += dh * (covered fraction) at every node over the finest-cell
+    rectangle [x0, x1) x [y0, y1)."""
+    eta = sim.get("A.eta0")
+    for lev in range(L + 1):
+        s = 1 << (L - lev)
+        n = 1 << lev
+        i = np.arange(n * n)
+        cx, cy = mo._compact(i), mo._compact(i >> 1)
+        ox = np.clip(np.minimum((cx + 1) * s, x1) - np.maximum(cx * s, x0), 0, None)
+        oy = np.clip(np.minimum((cy + 1) * s, y1) - np.maximum(cy * s, y0), 0, None)
+        base = ((1 << (2 * lev)) - 1) // 3 + 3
+        eta[base:base + n * n] += dh * (ox * oy) / float(s * s)
+    sim.set("A.eta0", eta)
+
+
+def test_learned_step_after_upload(case_dir):
+    """Water uploaded onto quiet dry units between steps: the learned step
+    re-encodes every parent and wakes every unit, so it makes the plain
+    step's tree and stays within 1e-3 m of it."""
+    import mwdg2
+    from bench import finest_fields
+    L = 8
+    par = cases.urban(str(case_dir / "lrn_upload8"), L=L, t_end=1800.0)
+    rect = (64, 128, 16, 48)
+    with mwdg2.Simulation(par, "f64") as ref:
+        ref.step(20)
+        _add_water(ref, L, *rect, 0.5)
+        ref.step(20)
+        h_ref = finest_fields(ref, L)[0]
+        leaves_ref = ref.get("leaves")
+    with mwdg2.Mlp(3) as m, mwdg2.Simulation(par, "f64") as s:
+        s.set_learned(m, decide_every=8, full_every=32, threshold=2.0)
+        s.step(20)
+        q0 = s.learn_counters()["quiet_unit_steps"]
+        _add_water(s, L, *rect, 0.5)
+        s.step(20)
+        h = finest_fields(s, L)[0]
+        leaves = s.get("leaves")
+    dh = float(np.max(np.abs(h - h_ref)))
+    print("quiet unit-steps before upload", q0, "dh", dh)
+    assert q0 > 0
+    assert np.array_equal(leaves, leaves_ref)
+    assert dh <= 1e-3
+    x0, x1, y0, y1 = rect
+    i = np.arange(4 ** L)
+    cx, cy = mo._compact(i), mo._compact(i >> 1)
+    inside = (cx >= x0) & (cx < x1) & (cy >= y0) & (cy < y1)
+    assert h_ref[inside].mean() > 0.3                  # the uploaded water is there and moving
