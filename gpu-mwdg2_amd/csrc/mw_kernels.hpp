@@ -6,6 +6,10 @@
 namespace mw {
 
 constexpr int WG = 256;          // work-group size (4 wave64s)
+// Minimum waves per SIMD asked of a latency-bound kernel's FP32 build (the
+// register cap that allows them); FP64 builds keep their registers (a cap
+// makes them spill).
+#define WAVES32(n) (sizeof(real) == 4 ? (n) : 1)
 constexpr int EMIT_ITEMS = 4;    // level-(L-1) nodes per thread in leaf emission
 constexpr int EMIT_TILE = WG * EMIT_ITEMS;
 constexpr int MAX_SRC = 4096;
