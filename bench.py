@@ -354,6 +354,7 @@ def main():
     t0 = time.time()
     sim = mwdg2.Simulation(par, "f32")
     t_setup = time.time() - t0
+    sweep_groups = sim.sweep_groups()
     sim.step(args.warmup)
     sim.sync()
     # the timed steps run without events; the kernel durations come from a
@@ -452,7 +453,7 @@ def main():
                    "mean_parents_encoded": (c1["parents_encoded"] - c0["parents_encoded"]) / max(1, steps),
                    "mean_parents_decoded": (c1["parents_decoded"] - c0["parents_decoded"]) / max(1, steps),
                    "mean_wet_leaves": [w / max(1, steps) for w in wet],
-                   "sim_time_reached": c2["time"], "setup_s": round(t_setup, 2),
+                   "sim_time_reached": c2["time"], "setup_s": round(t_setup, 2), "sweep_groups": sweep_groups,
                    "ms_per_step_with_kernel_events": round(ms_profiled, 4),
                    "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                for k, v in kern.items()}},

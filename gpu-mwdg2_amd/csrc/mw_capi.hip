@@ -258,6 +258,13 @@ struct Ctx : CtxBase {
     H = lvl_idx(L + 1);
     mwm = hc.solver == MWDG2;
     MW_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (mwm && !hc.limitslopes) {
+      int dev = 0, cus = 0, per_cu = 0;
+      MW_HIP(hipGetDevice(&dev));
+      MW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      MW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep2<real>, WG, 0));
+      resident_g = std::min(RED_SLOTS, std::max(8, cus * per_cu / 8 * 8));
+    }
     Hp = ((size_t)H + 3) / 4 * 4;
     for (int b = 0; b < 2; ++b) {
       hier[b].eta0 = alloc<real>(Hp);
@@ -679,7 +686,16 @@ struct Ctx : CtxBase {
     });
   }
   int sweep_g = 0;                            // work-groups of the leaf sweeps (0: the default)
-  int default_sweep_groups() const override { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
+  // The leaf sweeps loop over tiles, so their work-group count is free. The
+  // default is one resident round of the MWDG2 stage-2 sweep (CUs x its
+  // occupancy, a multiple of 8): a count between one and two rounds leaves a
+  // tail (urban 4096^2, FP32: 2048 groups 236 us, 1536 groups 200 us, 4096
+  // groups 208 us per launch).
+  int resident_g = 0;
+  int default_sweep_groups() const override {
+    const int g = std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS);
+    return resident_g ? std::min(g, resident_g) : g;
+  }
   int leaf_grid() const { return sweep_g ? sweep_g : default_sweep_groups(); }
   int sweep_groups() const override { return leaf_grid(); }
   // The sweeps' work-group count (a multiple of 8, 8 .. 4096). The per-group
@@ -2088,6 +2104,12 @@ int mw_learn_counters(mw_ctx* ctx, long long* out) {
 int mw_set_sweep_groups(mw_ctx* ctx, int groups) {
   MW_NEED(ctx);
   return guarded([&] { ctx->impl->set_sweep_groups(groups); });
+}
+
+int mw_get_sweep_groups(mw_ctx* ctx, int* groups) {
+  MW_NEED(ctx);
+  MW_NEED(groups);
+  return guarded([&] { *groups = ctx->impl->sweep_groups(); });
 }
 
 int mw_ensemble_create(mw_ctx* const* ctxs, int n, int autotune_steps, mw_ensemble** out) {

@@ -106,6 +106,7 @@ def lib():
         L.mw_set_learned.argtypes = [vp, vp, i, i, ctypes.c_float]
         L.mw_learn_counters.argtypes = [vp, lp]
         L.mw_set_sweep_groups.argtypes = [vp, i]
+        L.mw_get_sweep_groups.argtypes = [vp, ctypes.POINTER(i)]
         L.mw_ensemble_create.argtypes = [ctypes.POINTER(vp), i, i, ctypes.POINTER(vp)]
         L.mw_ensemble_step.argtypes = [vp, i]
         L.mw_ensemble_sync.argtypes = [vp]
@@ -198,6 +199,11 @@ class Simulation:
 
     def set_sweep_groups(self, groups: int):
         _check(lib().mw_set_sweep_groups(self._h, int(groups)))
+
+    def sweep_groups(self) -> int:
+        g = ctypes.c_int()
+        _check(lib().mw_get_sweep_groups(self._h, ctypes.byref(g)))
+        return g.value
 
     def learn_counters(self) -> dict:
         out = (ctypes.c_longlong * 4)()

@@ -259,8 +259,10 @@ int mw_learn_counters(mw_ctx* ctx, long long* out);
  * ensemble before its contexts. */
 typedef struct mw_ensemble mw_ensemble;
 /* Work-groups of the leaf sweeps (multiple of 8, 8..4096); results do not
- * depend on it. */
+ * depend on it. The default for MWDG2 without limiter is one resident round
+ * of the stage-2 sweep on the context's device (1536 on an MI355X in FP32). */
 int mw_set_sweep_groups(mw_ctx* ctx, int groups);
+int mw_get_sweep_groups(mw_ctx* ctx, int* groups);
 int mw_ensemble_create(mw_ctx* const* ctxs, int n, int autotune_steps, mw_ensemble** out);
 int mw_ensemble_step(mw_ensemble* e, int steps);
 int mw_ensemble_sync(mw_ensemble* e);

@@ -79,6 +79,7 @@ def test_learned_and_ensemble_argument_errors():
     assert lib.mw_set_learned(None, None, 8, 32, ctypes.c_float(0.5)) == 3
     assert lib.mw_learn_counters(None, None) == 3
     assert lib.mw_set_sweep_groups(None, 64) == 3
+    assert lib.mw_get_sweep_groups(None, None) == 3
     h = ctypes.c_void_p()
     assert lib.mw_ensemble_create(None, 0, 0, ctypes.byref(h)) == 3
     assert lib.mw_ensemble_step(None, 1) == 3

@@ -36,6 +36,8 @@ def test_sweep_groups_do_not_change_results(case_dir):
         a.step(2)
         b.step(2)
         b.set_sweep_groups(8)
+        g = a.sweep_groups()                # default: at most one group per 256 finest cells, one resident round
+        assert g % 8 == 0 and 8 <= g <= 256 and b.sweep_groups() == 8
         a.step(8)
         b.step(8)
         _same(a, b, "groups")
