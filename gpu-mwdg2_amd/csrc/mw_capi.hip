@@ -154,10 +154,12 @@ struct CtxBase {
   virtual void set_sweep_groups(int g) = 0;
   virtual int sweep_groups() const = 0;
   virtual int default_sweep_groups() const = 0;
+  virtual int tile_groups() const = 0;           // one work-group per 256-leaf tile of the finest level (capped)
   virtual void capture_step(hipGraph_t* g) = 0;
   virtual int parity() const = 0;
   virtual void set_parity(int c) = 0;
   virtual void check_graphable() const = 0;
+  virtual void ensemble_step() = 0;               // one step of the captured sequence, launched directly
   bool in_ensemble = false;
 };
 
@@ -692,8 +694,9 @@ struct Ctx : CtxBase {
   // tail (urban 4096^2, FP32: 2048 groups 236 us, 1536 groups 200 us, 4096
   // groups 208 us per launch).
   int resident_g = 0;
+  int tile_groups() const override { return std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS); }
   int default_sweep_groups() const override {
-    const int g = std::min((nblk(N) + 7) / 8 * 8, RED_SLOTS);
+    const int g = tile_groups();
     return resident_g ? std::min(g, resident_g) : g;
   }
   int leaf_grid() const { return sweep_g ? sweep_g : default_sweep_groups(); }
@@ -1300,6 +1303,10 @@ struct Ctx : CtxBase {
   std::string out_path() const override { return hc.dirroot + "/" + hc.resroot + ".stage"; }
   bool voutput_stage() const override { return hc.voutput_stage; }
 
+  void ensemble_step() override {
+    one_step();
+    MW_HIP(hipGetLastError());
+  }
   void step(int n) override {
     if (in_ensemble) throw std::invalid_argument("the context belongs to an ensemble: step the ensemble");
     for (int i = 0; i < n; ++i) one_step();
@@ -1809,8 +1816,10 @@ struct Ensemble {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<hipGraph_t> child;
   std::vector<hipGraphExec_t> own;               // [scenario][2]: each scenario's graphs, instantiated
-  int mode = 0;                                  // 0: parent graph, 1: a graph per scenario stream
-  double mode_ms[2] = {0.0, 0.0};
+  // 0: parent graph, 1: a graph per scenario stream, 2: the captured
+  // sequence launched directly on each scenario's stream
+  int mode = 0;
+  double mode_ms[3] = {0.0, 0.0, 0.0};
   long long k = 0;
 
   ~Ensemble() {
@@ -1843,6 +1852,17 @@ struct Ensemble {
     MW_HIP(hipEventElapsedTime(&ms, a.e, b.e));
     return ms / n;
   }
+  // ms per step of all scenarios stepped one after another on their own
+  // streams (before capture)
+  double time_together(int n) {
+    for (CtxBase* c : ctx) c->step(1);
+    for (CtxBase* c : ctx) MW_HIP(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i)
+      for (CtxBase* c : ctx) c->step(1);
+    for (CtxBase* c : ctx) MW_HIP(hipStreamSynchronize(c->stream));
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n;
+  }
   void create(const std::vector<CtxBase*>& cs, int autotune) {
     ctx = cs;
     for (CtxBase* c : ctx) {
@@ -1855,9 +1875,11 @@ struct Ensemble {
       int g = c->sweep_groups();
       double best = 0.0;
       if (autotune > 0) {
+        // one resident round of the sweep and fewer (room for the other
+        // scenarios' kernels), and two rounds
         const int g0 = c->default_sweep_groups();
-        for (int cand : {g0, g0 / 2, g0 / 4, g0 / 8}) {
-          if (cand < 8 || cand % 8) continue;
+        for (int cand : {g0, g0 / 2, g0 / 4, g0 / 8, 2 * g0}) {
+          if (cand < 8 || cand % 8 || cand > c->tile_groups()) continue;
           c->set_sweep_groups(cand);
           c->step(1);
           const double ms = time_steps(*c, autotune);
@@ -1868,6 +1890,23 @@ struct Ensemble {
       groups.push_back(g);
       tune_ms.push_back(best);
       MW_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (autotune > 0) {
+      // each count above was timed with its scenario alone: keep the choice
+      // only if the scenarios stepped together run at least 2% faster with it
+      // than with the default counts
+      std::vector<int> def;
+      for (CtxBase* c : ctx) def.push_back(c->default_sweep_groups());
+      if (def != groups) {
+        const double t_tuned = time_together(std::max(8, 2 * autotune));
+        for (size_t i = 0; i < ctx.size(); ++i) ctx[i]->set_sweep_groups(def[i]);
+        const double t_def = time_together(std::max(8, 2 * autotune));
+        if (t_tuned < 0.98 * t_def) {
+          for (size_t i = 0; i < ctx.size(); ++i) ctx[i]->set_sweep_groups(groups[i]);
+        } else {
+          groups = def;
+        }
+      }
     }
     for (int p = 0; p < 2; ++p) {
       MW_HIP(hipGraphCreate(&parent[p], 0));
@@ -1893,25 +1932,33 @@ struct Ensemble {
       own.push_back(x);
     }
     if (autotune > 0) {
-      for (int m = 0; m < 2; ++m) {
+      for (int m = 0; m < 3; ++m) {
         mode = m;
         step(1);
         wait();
+        const int n = std::max(8, 2 * autotune);
         const auto t0 = std::chrono::steady_clock::now();
-        step(2 * autotune);
+        step(n);
         wait();
-        mode_ms[m] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
-                     (2 * autotune);
+        mode_ms[m] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n;
       }
-      mode = mode_ms[1] < mode_ms[0] ? 1 : 0;
+      mode = 0;
+      for (int m = 1; m < 3; ++m)
+        if (mode_ms[m] < mode_ms[mode]) mode = m;
     }
   }
   void step(int n) {
     for (int i = 0; i < n; ++i) {
       if (mode == 0) {
         MW_HIP(hipGraphLaunch(exec[k & 1], stream));
-      } else {
+      } else if (mode == 1) {
         for (size_t j = 0; j < ctx.size(); ++j) MW_HIP(hipGraphLaunch(own[2 * j + (k & 1)], ctx[j]->stream));
+      } else {
+        // the host mask parity of step k, as the graph of that parity starts from
+        for (size_t j = 0; j < ctx.size(); ++j) {
+          ctx[j]->set_parity((k & 1) ? par1[j] : par0[j]);
+          ctx[j]->ensemble_step();
+        }
       }
       ++k;
     }
@@ -2151,15 +2198,13 @@ int mw_ensemble_groups(mw_ensemble* e, int* groups, double* ms_per_step) {
 int mw_ensemble_launch_mode(mw_ensemble* e, int* mode, double* ms_per_step) {
   MW_NEED(e && mode);
   *mode = e->impl.mode;
-  if (ms_per_step) {
-    ms_per_step[0] = e->impl.mode_ms[0];
-    ms_per_step[1] = e->impl.mode_ms[1];
-  }
+  if (ms_per_step)
+    for (int m = 0; m < 3; ++m) ms_per_step[m] = e->impl.mode_ms[m];
   return MW_OK;
 }
 
 int mw_ensemble_set_launch_mode(mw_ensemble* e, int mode) {
-  MW_NEED(e && (mode == 0 || mode == 1));
+  MW_NEED(e && mode >= 0 && mode <= 2);
   return guarded([&] {
     e->impl.wait();
     e->impl.mode = mode;

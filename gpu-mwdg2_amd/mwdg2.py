@@ -423,7 +423,7 @@ class Ensemble:
 
     def launch_mode(self):
         m = ctypes.c_int()
-        ms = (ctypes.c_double * 2)()
+        ms = (ctypes.c_double * 3)()
         _check(lib().mw_ensemble_launch_mode(self._h, ctypes.byref(m), ms))
         return m.value, list(ms)
 

@@ -252,8 +252,10 @@ int mw_learn_counters(mw_ctx* ctx, long long* out);
  * child graphs of one parent graph, and one ensemble step is one graph launch;
  * every scenario keeps its own time step. mw_ensemble_create runs each
  * scenario's first step (and, with autotune_steps > 0, times autotune_steps
- * steps at each candidate sweep work-group count and keeps the fastest, then
- * the launch shape, see mw_ensemble_launch_mode).
+ * steps at each candidate sweep work-group count and keeps the fastest, keeps
+ * those counts only if all scenarios stepped together run faster with them
+ * than with the defaults, then times the launch shape, see
+ * mw_ensemble_launch_mode).
  * Between create and destroy the contexts are stepped only through the
  * ensemble; after mw_ensemble_sync their arrays may be read. Destroy the
  * ensemble before its contexts. */
@@ -270,8 +272,10 @@ int mw_ensemble_sync(mw_ensemble* e);
  * autotuning) of each scenario. */
 int mw_ensemble_groups(mw_ensemble* e, int* groups, double* ms_per_step);
 /* Launch shape: 0 = one launch of the parent graph per step, 1 = each
- * scenario's graph on its own stream. With autotuning, create times both
- * (ms_per_step[2]) and keeps the faster; set_launch_mode overrides it. */
+ * scenario's graph on its own stream, 2 = each scenario's captured kernel
+ * sequence launched directly on its own stream (no graph; same kernels and
+ * arguments). With autotuning, create times all three (ms_per_step[3]) and
+ * keeps the fastest; set_launch_mode overrides it. */
 int mw_ensemble_launch_mode(mw_ensemble* e, int* mode, double* ms_per_step);
 int mw_ensemble_set_launch_mode(mw_ensemble* e, int mode);
 int mw_ensemble_destroy(mw_ensemble* e);

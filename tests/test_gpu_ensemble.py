@@ -69,6 +69,19 @@ def test_ensemble_replays_match_eager_steps(case_dir):
         for i, (a, b) in enumerate(zip(ens_sims, twins)):
             b.step(1)
             _same(a, b, ("streams", r, i))
+    ens.set_launch_mode(2)                          # the same kernels launched directly, both parities
+    for r in range(3):
+        ens.step(1 + r)
+        ens.sync()
+        for i, (a, b) in enumerate(zip(ens_sims, twins)):
+            b.step(1 + r)
+            _same(a, b, ("direct", r, i))
+    ens.set_launch_mode(0)                          # back to the parent graph after an odd count
+    ens.step(3)
+    ens.sync()
+    for i, (a, b) in enumerate(zip(ens_sims, twins)):
+        b.step(3)
+        _same(a, b, ("graph again", i))
     ens.close()
     for s in ens_sims + twins:
         s.close()
@@ -83,7 +96,7 @@ def test_autotuned_ensemble_matches_eager(case_dir):
     g, ms = ens.groups()
     assert all(x % 8 == 0 and 8 <= x <= 4096 for x in g) and all(m > 0 for m in ms)
     mode, mode_ms = ens.launch_mode()
-    assert mode in (0, 1) and min(mode_ms) > 0
+    assert mode in (0, 1, 2) and min(mode_ms) > 0
     for a, b in zip(ens_sims, twins):
         b.step(a.counters()["steps"])
     ens.step(4)
