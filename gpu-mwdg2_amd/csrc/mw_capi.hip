@@ -1361,6 +1361,9 @@ struct Ctx : CtxBase {
   }
 
   void stage(const std::string& s) override {
+    if (learn || tr_x)
+      throw std::logic_error("single stages run the plain step: turn the learned step and trace recording off first");
+    if (in_ensemble) throw std::logic_error("the context belongs to an ensemble: step the ensemble");
     if (s == "prologue") k_prologue_launch();
     else if (s == "point_sources") k_sources_launch();
     else if (s == "encode") encode_launch();
@@ -1441,8 +1444,12 @@ struct Ctx : CtxBase {
     if (!dev) throw std::invalid_argument("unknown or absent array " + name);
     if (host && count > 0) {
       const size_t bytes = (size_t)std::min(count, len) * esz;
-      if (up) MW_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
-      else MW_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+      if (up) {
+        learn_invalidate();
+        MW_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
+      } else {
+        MW_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
+      }
       MW_HIP(hipStreamSynchronize(stream));
     }
     return len;

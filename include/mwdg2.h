@@ -123,7 +123,10 @@ int mw_update_stage(mw_ctx* ctx, int stage); /* 1: load_soln+ghosts+friction+(li
 int mw_encode_scale(mw_ctx* ctx);        /* reinsert + encode_flow(for_nghbrs=true)               */
 int mw_cfl_min(mw_ctx* ctx, double* dt_next); /* operators/get_dt_CFL.cu:3-50 (reads the device min) */
 /* Any of the above by name: "prologue", "point_sources", "encode", "decode",
- * "emit", "swap_masks", "stage1", "fv1", "encode_scale", "stage2". */
+ * "emit", "swap_masks", "stage1", "fv1", "encode_scale", "stage2".
+ * The single stages are the plain step's: they return MW_ERR_STATE while
+ * the learned step or trace recording is on, or the context is in an
+ * ensemble. */
 int mw_stage_named(mw_ctx* ctx, const char* name);
 
 /* Named device arrays: hierarchy "A.eta0".."A.qy1y", "B.*", "z0","z1x","z1y";
@@ -133,6 +136,8 @@ int mw_stage_named(mw_ctx* ctx, const char* name);
  * the .par keyword vtk; operators/dg2_update.cu:61,357, fv1_update.cu:51,237).
  * count is in elements; *len (optional) receives the array length. */
 int mw_download(mw_ctx* ctx, const char* name, void* host, long long count, long long* len);
+/* With the learned step on, an upload makes the next step re-encode every
+ * parent and sweep every unit (its records of unchanged work are dropped). */
 int mw_upload(mw_ctx* ctx, const char* name, const void* host, long long count);
 /* Scalars: "time","dt","dt_next","n_leaves","first","mx_eta","mx_h","mx_qx","mx_qy","mx_z". */
 int mw_set_scalar(mw_ctx* ctx, const char* name, double v);

@@ -246,6 +246,9 @@ def test_learned_step_after_upload(case_dir):
         s.set_learned(m, decide_every=8, full_every=32, threshold=2.0)
         s.step(20)
         q0 = s.learn_counters()["quiet_unit_steps"]
+        with pytest.raises(mwdg2.MWError) as e:        # single stages are the plain step's
+            s.stage("encode")
+        assert e.value.code == 4
         _add_water(s, L, *rect, 0.5)
         s.step(20)
         h = finest_fields(s, L)[0]
