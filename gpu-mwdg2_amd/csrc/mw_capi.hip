@@ -880,16 +880,7 @@ struct Ctx : CtxBase {
     utouch = alloc<uint8_t>(N_UNITS);
     uborder = alloc<uint8_t>(N_UNITS);
     umx = alloc<real>((size_t)4 * N_UNITS);
-    {
-      // units touching the domain border or reaching past the DEM
-      std::vector<uint8_t> b(N_UNITS);
-      const int side = 1 << (L - UNIT_LEVEL);
-      for (int u = 0; u < N_UNITS; ++u) {
-        const int x0 = compact(u) * side, y0 = compact(u >> 1) * side;
-        b[u] = (x0 == 0 || y0 == 0 || x0 + side - 1 >= hc.xsz - 1 || y0 + side - 1 >= hc.ysz - 1) ? 1 : 0;
-      }
-      MW_HIP(hipMemcpy(uborder, b.data(), N_UNITS, hipMemcpyHostToDevice));
-    }
+    set_border_units();
     MW_HIP(hipMemsetAsync(umode, 1, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uchg, 0, N_UNITS, stream));
     MW_HIP(hipMemsetAsync(uage, 0, sizeof(int) * N_UNITS, stream));
@@ -972,6 +963,24 @@ struct Ctx : CtxBase {
     MW_HIP(hipMemsetAsync(uwet, 1, N_UNITS, stream));      // no unit is known dry yet
     MW_HIP(hipMemsetAsync(utouch, 1, N_UNITS, stream));
     set_quiet_params();
+  }
+  // Units that may never be quiet: those reaching past the DEM and, unless
+  // a raster case has walls on every side (a ghost cell is then a function
+  // of its border cell alone, so an unchanged dry border cell stays
+  // unchanged), those touching the domain border.
+  void set_border_units() {
+    if (!uborder) return;
+    bool walls = hc.test_case == 0;
+    for (int k = 0; k < 4; ++k) walls = walls && hc.bdy[k].type == CLOSED;
+    std::vector<uint8_t> b(N_UNITS);
+    const int side = 1 << (L - UNIT_LEVEL);
+    for (int u = 0; u < N_UNITS; ++u) {
+      const int x0 = compact(u) * side, y0 = compact(u >> 1) * side;
+      const bool past = x0 + side > hc.xsz || y0 + side > hc.ysz;
+      const bool edge = x0 == 0 || y0 == 0 || x0 + side - 1 >= hc.xsz - 1 || y0 + side - 1 >= hc.ysz - 1;
+      b[u] = (walls ? past : edge) ? 1 : 0;
+    }
+    MW_HIP(hipMemcpy(uborder, b.data(), N_UNITS, hipMemcpyHostToDevice));
   }
   // After the state changed outside the step (new initial state, uploads):
   // every parent is re-encoded and no unit is quiet until swept again.
@@ -1535,6 +1544,7 @@ struct Ctx : CtxBase {
   void rebuild_all() {
     MW_HIP(hipStreamSynchronize(stream));
     learn_invalidate();
+    set_border_units();
     clear_hierarchies();
     for (uint8_t* p : {sig[0], sig[1], xflag, leaflev, preflag}) MW_HIP(hipMemsetAsync(p, 0, D + 4, stream));
     fill_params();
