@@ -229,6 +229,7 @@ def learned_leg(par: str, L: int, warmup: int, steps: int, trace_steps: int) -> 
             sim.profile(0)
             lc = sim.learn_counters()
             r["frozen_unit_fraction"] = round(lc["frozen_unit_steps"] / max(1, lc["steps"] * 4096), 4)
+            r["quiet_unit_fraction"] = round(lc["quiet_unit_steps"] / max(1, lc["steps"] * 4096), 4)
             r["parents_kept_per_step"] = lc["parents_kept"] / max(1, lc["steps"])
             h, qx, qy, v = finest_fields(sim, L)
             fields = (h, qx, qy, v, sim.counters()["time"])
